@@ -1,0 +1,247 @@
+"""Benchmark: FP64 brick stencil sweeps at 512^3 on one MI355X.
+
+Workload (the headline): 7-point star, FP64, 512^3 interior cells plus an
+8-cell ghost zone (66^3 bricks of 8x8x8), inputs U[-1,1], per-axis
+symmetric coefficients U(0,1) normalised to sum 1, periodic ghost-brick
+refresh before every sweep, ping-pong sweeps.  One "step" is one sweep
+(refresh + stencil) over all 262144 interior bricks; the reported value is
+interior cells per second of the timed steps (GStencil/s), timed with HIP
+events recorded on the stream the kernels run on.
+
+The same JSON line also carries:
+  roofline     -- the sweep kernel alone (no refresh): algorithmic bytes
+                  (16 B/point: one FP64 read + one write) / its mean launch
+                  time, against the 8 TB/s HBM3E peak; `measured_copy_gbs`
+                  is a plain device copy kernel on the same box;
+  cpu_baseline -- the oracle's restatement of the reference brick loop
+                  (stencils/3axis.cpp:51-66) with OpenMP on the host cores;
+  by_stencil   -- 13/25-point, 125-point cube and cond at the same size,
+                  plus the FP32 / BF16 forms of the 7-point star.
+
+Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--n", type=int, default=512, help="interior cells per axis (multiple of 8)")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--only", default="", help="comma list of extra cases to run (default all)")
+    p.add_argument("--no-extra", action="store_true")
+    return p.parse_args()
+
+
+def time_sweeps(bk, st, a, b, g, stream, steps, warmup, refresh=True):
+    """ms per sweep over `steps` ping-pong sweeps (events on `stream`)."""
+    from bricklib_amd import Event
+    if warmup:
+        bk.sweeps(st, a, b, g, warmup, refresh=refresh, stream=stream)
+    stream.sync()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    bk.sweeps(st, a, b, g, steps, refresh=refresh, stream=stream)
+    e1.record(stream)
+    e1.sync()
+    return e0.elapsed_ms(e1) / steps
+
+
+def copy_peak(bk, stream, nbytes=2 << 30, reps=20):
+    """Measured HBM bandwidth of the library's brick copy kernel (read +
+    write bytes / time) over `nbytes` of FP64 bricks."""
+    from bricklib_amd import DeviceBuffer, Event
+    from bricklib_amd import _lib
+    import ctypes as C
+    nbr = nbytes // 4096
+    buf = DeviceBuffer(2 * nbr * 4096)
+    pairs = np.empty(2 * nbr, np.uint32)
+    pairs[0::2] = np.arange(nbr, 2 * nbr, dtype=np.uint32)
+    pairs[1::2] = np.arange(nbr, dtype=np.uint32)
+    dp = DeviceBuffer.from_array(pairs)
+    lib = bk.load()
+
+    def run():
+        _lib.check(lib.bk_ghost_refresh(0, C.c_void_p(buf.ptr), C.c_size_t(512),
+                                        C.c_size_t(512), C.c_void_p(dp.ptr), C.c_uint32(nbr),
+                                        C.c_void_p(int(stream))))
+    run()
+    stream.sync()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(reps):
+        run()
+    e1.record(stream)
+    e1.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    return 2 * nbr * 4096 / (ms * 1e-3) / 1e9, ms
+
+
+def cpu_baseline(nb, coeff, seconds):
+    """Oracle brick loop (reference order, 3axis.cpp:51-66) on the host."""
+    import oracle as O
+    import ctypes as C
+    g = O.Grid(nb)
+    a = g.storage()
+    b = g.storage()
+    g.fill_uniform(a, 1234)
+    g.refresh(a)
+    c = np.ascontiguousarray(coeff)
+    fn = O.lib().or_brick_sweep_7pt_accessor
+
+    def sweep(src, dst):
+        fn(O.P(g.adj, O.u32p), O.P(g.interior, O.u32p), g.interior.size, O.P(src, O.dblp), 512,
+           O.P(dst, O.dblp), 512, O.P(c, O.dblp))
+    sweep(a, b)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    src, dst = a, b
+    while True:
+        g.refresh(src)
+        sweep(src, dst)
+        src, dst = dst, src
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 50:
+            break
+    pts = g.interior.size * 512
+    return {"value": pts * n / el / 1e9, "unit": "GStencil/s",
+            "cores": int(O.lib().or_num_threads()), "kind": "port",
+            "sample": f"{n} sweeps (periodic refresh + 7-pt brick loop) over the full "
+                      f"{8 * nb[0]}^3 FP64 grid, {el:.1f} s"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
+              "(see the sharding section of DESIGN.md) and is not measured", file=sys.stderr)
+        sys.exit(2)
+    import bricklib_amd as bk
+    from bricklib_amd import stencils as S
+
+    if bk.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    n = args.n
+    nb = (n // 8,) * 3
+    stream = bk.Stream()
+    t_setup = time.perf_counter()
+    g = bk.PeriodicGrid(nb, stream=stream)
+    pts = g.cells
+    stream.sync()
+
+    def fields(dtype):
+        sa, sb = g.field_storage(dtype), g.field_storage(dtype)
+        a, b = bk.Brick(g.info, sa, 0), bk.Brick(g.info, sb, 0)
+        g.fill_uniform(a, 1234, stream=stream)
+        g.fill_uniform(b, 4321, stream=stream)
+        return a, b
+
+    a, b = fields(bk.BK_F64)
+    st7 = S.Stencil.make("7pt", seed=1)
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s, {g.nbricks} bricks, "
+          f"{g.npairs} ghost bricks", file=sys.stderr, flush=True)
+
+    # headline: refresh + sweep, K timed steps
+    ms_step = time_sweeps(bk, st7, a, b, g, stream, args.steps, args.warmup)
+    # the dominant kernel alone, for the roofline
+    ms_kernel = time_sweeps(bk, st7, a, b, g, stream, args.steps, 2, refresh=False)
+    peak_copy, _ = copy_peak(bk, stream)
+    alg_bytes = 16 * pts
+    achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
+    print(f"[bench] 7pt f64: {ms_step:.4f} ms/step, kernel {ms_kernel:.4f} ms, "
+          f"{achieved:.0f} GB/s, copy peak {peak_copy:.0f} GB/s", file=sys.stderr, flush=True)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"star7_f64_{n}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    by = {}
+    if not args.no_extra:
+        only = set(x for x in args.only.split(",") if x)
+        cases = [("13pt_f64", "13pt", bk.BK_F64), ("25pt_f64", "25pt", bk.BK_F64),
+                 ("125pt_f64", "125pt", bk.BK_F64), ("cond_f64", "cond", bk.BK_F64),
+                 ("7pt_f32", "7pt", bk.BK_F32), ("7pt_bf16", "7pt", bk.BK_BF16),
+                 ("125pt_f32", "125pt", bk.BK_F32)]
+        cur = {bk.BK_F64: (a, b)}
+        for name, sname, dt in cases:
+            if only and name not in only:
+                continue
+            if dt not in cur:
+                cur = {bk.BK_F64: (a, b), dt: fields(dt)}
+            fa, fb = cur[dt]
+            st = S.Stencil.make(sname, seed=1)
+            steps = max(10, args.steps // (5 if sname == "125pt" else 1))
+            ms_s = time_sweeps(bk, st, fa, fb, g, stream, steps, 2)
+            ms_k = time_sweeps(bk, st, fa, fb, g, stream, steps, 1, refresh=False)
+            es = {bk.BK_F64: 8, bk.BK_F32: 4, bk.BK_BF16: 2}[dt]
+            fl = S.flops_per_point(st.kind)
+            gb = 2 * es * pts / (ms_k * 1e-3) / 1e9
+            tf = fl * pts / (ms_k * 1e-3) / 1e12
+            peak_tf = FP64_PEAK_TFLOPS if dt == bk.BK_F64 else FP32_PEAK_TFLOPS
+            by[name] = {"gstencil_s": round(pts / (ms_s * 1e-3) / 1e9, 2),
+                        "kernel_gstencil_s": round(pts / (ms_k * 1e-3) / 1e9, 2),
+                        "ms_per_step": round(ms_s, 4), "kernel_ms": round(ms_k, 4),
+                        "hbm_gbs": round(gb, 1), "hbm_frac": round(gb / HBM_PEAK_GBS, 4),
+                        "tflops": round(tf, 2), "valu_frac": round(tf / peak_tf, 4)}
+            print(f"[bench] {name}: {by[name]}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(nb, st7.coeff, args.cpu_seconds)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    value = pts / (ms_step * 1e-3) / 1e9
+    line = {
+        "metric": "GStencil/s + % HBM roofline at 512^3 (7pt/125pt); p99 latency batched solves",
+        "value": round(value, 3),
+        "unit": "GStencil/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U[-1,1] inputs, seeded coefficients)",
+        "config": {"workload": f"7-point star FP64, {n}^3 interior + 8-cell ghost, 8x8x8 bricks "
+                               f"(fold 8x8), periodic ghost-brick refresh every sweep, "
+                               f"ping-pong sweeps", "interior_cells": pts,
+                   "bricks": g.nbricks, "ghost_bricks": g.npairs},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel_ms": round(ms_kernel, 5),
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "measured_copy_gbs": round(peak_copy, 1),
+                     "frac_of_measured_copy": round(achieved / peak_copy, 4)},
+        "cpu_baseline": cpu,
+        "by_stencil": by,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+"""ctypes binding of libbricks.so (the C-ABI declared in include/bk_api.h).
+
+Loading fails loudly: there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbricks.so")
+
+BK_OK = 0
+BK_ERR_ARG = 1
+BK_ERR_NOMEM = 2
+BK_ERR_HIP = 3
+BK_ERR_UNSUPPORTED = 4
+BK_ERR_NODEV = 5
+
+BK_F64, BK_F32, BK_BF16 = 0, 1, 2
+BK_STAR7, BK_STAR13, BK_STAR25, BK_CUBE125, BK_COND7 = 0, 1, 2, 3, 4
+BK_COND_REFERENCE, BK_COND_THRESHOLD = 0, 1
+
+u32p = C.POINTER(C.c_uint32)
+dblp = C.POINTER(C.c_double)
+longp = C.POINTER(C.c_long)
+vp = C.c_void_p
+
+
+class bk_sweep_desc(C.Structure):
+    _fields_ = [
+        ("stencil", C.c_int),
+        ("dtype", C.c_int),
+        ("d_adj", vp),
+        ("d_list", vp),
+        ("nlist", C.c_uint32),
+        ("d_in", vp),
+        ("in_step", C.c_size_t),
+        ("d_out", vp),
+        ("out_step", C.c_size_t),
+        ("h_coeff", dblp),
+        ("theta", C.c_double),
+        ("cond_mode", C.c_int),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "bk_version": (C.c_int, []),
+    "bk_last_error": (C.c_char_p, []),
+    "bk_stencil_ncoeff": (C.c_int, [C.c_int]),
+    "bk_stencil_radius": (C.c_int, [C.c_int]),
+    "bk_dtype_size": (C.c_int, [C.c_int]),
+    "bk_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "bk_set_device": (C.c_int, [C.c_int]),
+    "bk_device_sync": (C.c_int, []),
+    "bk_storage_alloc": (C.c_int, [C.c_int, C.c_long, C.c_size_t, C.POINTER(vp)]),
+    "bk_free": (C.c_int, [vp]),
+    "bk_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),
+    "bk_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "bk_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "bk_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "bk_zero": (C.c_int, [vp, C.c_size_t, vp]),
+    "bk_stream_create": (C.c_int, [C.POINTER(vp)]),
+    "bk_stream_destroy": (C.c_int, [vp]),
+    "bk_stream_sync": (C.c_int, [vp]),
+    "bk_event_create": (C.c_int, [C.POINTER(vp)]),
+    "bk_event_destroy": (C.c_int, [vp]),
+    "bk_event_record": (C.c_int, [vp, vp]),
+    "bk_event_sync": (C.c_int, [vp]),
+    "bk_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+    "bk_init_grid": (C.c_int, [longp, C.c_int, u32p, u32p]),
+    "bk_ghost_pairs": (C.c_int, [longp, u32p, u32p]),
+    "bk_interior_list": (C.c_int, [longp, u32p, u32p]),
+    "bk_sweep": (C.c_int, [C.POINTER(bk_sweep_desc), vp]),
+    "bk_ghost_refresh": (C.c_int, [C.c_int, vp, C.c_size_t, C.c_size_t, vp, C.c_uint32, vp]),
+    "bk_sweeps": (C.c_int, [C.POINTER(bk_sweep_desc), vp, C.c_uint32, C.c_int, C.c_int, vp]),
+    "bk_graph_cache_clear": (C.c_int, []),
+    "bk_fill_uniform": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_uint64, C.c_double,
+                                  C.c_double, vp]),
+    "bk_field_stats": (C.c_int, [C.c_int, vp, C.c_size_t, vp, C.c_uint32, C.c_size_t, dblp, vp]),
+    "bk_field_compare": (C.c_int, [C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_size_t, vp,
+                                   C.c_uint32, C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                   dblp, vp]),
+    "bk_copy_to_brick": (C.c_int, [C.c_int, longp, longp, longp, vp, vp, vp, C.c_size_t, vp]),
+    "bk_copy_from_brick": (C.c_int, [C.c_int, longp, longp, longp, vp, vp, vp, C.c_size_t, vp]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libbricks.so, raising ImportError with the build command when it
+    is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m bricklib_amd.build` "
+            "(the HIP library is required: there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class BrickError(RuntimeError):
+    """A libbricks call returned a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[bk status {code}] {msg}")
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != BK_OK:
+        msg = load().bk_last_error()
+        raise BrickError(rc, msg.decode() if msg else "")
+
+
+def longs(v) -> C.Array:
+    return (C.c_long * 3)(*[int(x) for x in v])

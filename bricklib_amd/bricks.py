@@ -1,0 +1,405 @@
+"""Python mirror of bricklib's host interface over libbricks.so.
+
+Names follow the reference (CtopCsUtahEdu/bricklib) so code and tests read
+like its drivers: ``init_grid`` (bricksetup.h:73-89), ``BrickInfo``
+(brick.h:96-127), ``BrickStorage.allocate`` (brick.h:68-75, here resident in
+HBM), ``Brick`` (brick.h:353-395, a field view: storage + offset),
+``movBrickInfo`` / ``movBrickStorage`` (brick-gpu.h:42-103),
+``copyToBrick`` / ``copyFromBrick`` (bricksetup.h:172-221).  Every GPU
+operation goes through the C-ABI; nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BK_BF16, BK_F32, BK_F64, bk_sweep_desc, check, load, longs)
+from .stencils import Stencil, stencil_id
+
+DTYPE_NP = {BK_F64: np.float64, BK_F32: np.float32, BK_BF16: np.uint16}
+DTYPE_SIZE = {BK_F64: 8, BK_F32: 4, BK_BF16: 2}
+BRICK_ELEMS = 512  # 8 x 8 x 8
+
+
+def _vp(x) -> C.c_void_p:
+    if x is None:
+        return C.c_void_p(None)
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(int(x))
+
+
+# ---------------------------------------------------------------------------
+# device memory, streams, events
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """An HBM allocation owned by Python (freed with the object)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().bk_alloc(C.c_size_t(self.nbytes), C.byref(p)))
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray, stream=None) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        buf.upload(arr, stream)
+        return buf
+
+    def upload(self, arr: np.ndarray, stream=None, offset: int = 0) -> None:
+        arr = np.ascontiguousarray(arr)
+        if offset + arr.nbytes > self.nbytes:
+            raise ValueError("upload past the end of the buffer")
+        check(load().bk_memcpy_h2d(_vp(self.ptr + offset), _vp(arr), C.c_size_t(arr.nbytes),
+                                   _vp(stream)))
+
+    def download(self, dtype, count: int | None = None, stream=None, offset: int = 0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        if offset + out.nbytes > self.nbytes:
+            raise ValueError("download past the end of the buffer")
+        check(load().bk_memcpy_d2h(_vp(out), _vp(self.ptr + offset), C.c_size_t(out.nbytes),
+                                   _vp(stream)))
+        return out
+
+    def zero(self, stream=None) -> None:
+        check(load().bk_zero(_vp(self.ptr), C.c_size_t(self.nbytes), _vp(stream)))
+
+    def free(self) -> None:
+        if getattr(self, "ptr", 0):
+            load().bk_free(_vp(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream (default priority)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(load().bk_stream_create(C.byref(h)))
+        self.handle = h.value
+
+    def sync(self) -> None:
+        check(load().bk_stream_sync(_vp(self.handle)))
+
+    def __int__(self):
+        return int(self.handle or 0)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().bk_stream_destroy(_vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        check(load().bk_event_create(C.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None) -> None:
+        check(load().bk_event_record(_vp(self.handle), _vp(stream)))
+
+    def sync(self) -> None:
+        check(load().bk_event_sync(_vp(self.handle)))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(load().bk_event_elapsed_ms(_vp(self.handle), _vp(end.handle), C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().bk_event_destroy(_vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    n = C.c_int()
+    rc = load().bk_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def device_sync() -> None:
+    check(load().bk_device_sync())
+
+
+def _stream_of(stream):
+    if stream is None:
+        return None
+    return int(stream)
+
+
+# ---------------------------------------------------------------------------
+# BrickInfo / init_grid / BrickStorage / Brick
+# ---------------------------------------------------------------------------
+class BrickInfo:
+    """BrickInfo<3>: the adjacency list adj[nbricks][27] (uint32), on the
+    host (``adj``) and, after movBrickInfo, in HBM (``dev``)."""
+
+    def __init__(self, adj: np.ndarray):
+        self.adj = np.ascontiguousarray(adj, dtype=np.uint32).reshape(-1, 27)
+        self.dev: DeviceBuffer | None = None
+
+    @property
+    def nbricks(self) -> int:
+        return self.adj.shape[0]
+
+    @property
+    def dptr(self) -> int:
+        if self.dev is None:
+            raise RuntimeError("BrickInfo is not on the device; call movBrickInfo first")
+        return self.dev.ptr
+
+
+def init_grid(dims, periodic: bool = False):
+    """init_grid<3>(grid_ptr, dims) -> (grid, BrickInfo).  ``dims`` are brick
+    counts contiguous-first (i, j, k); ``grid`` has shape (k, j, i)."""
+    dims = [int(d) for d in dims]
+    n = dims[0] * dims[1] * dims[2]
+    grid = np.empty(n, dtype=np.uint32)
+    adj = np.empty(n * 27, dtype=np.uint32)
+    check(load().bk_init_grid(longs(dims), int(bool(periodic)),
+                              grid.ctypes.data_as(_lib.u32p), adj.ctypes.data_as(_lib.u32p)))
+    return grid.reshape(dims[2], dims[1], dims[0]), BrickInfo(adj)
+
+
+def movBrickInfo(info: BrickInfo, stream=None) -> BrickInfo:
+    """Copy the adjacency list into HBM (brick-gpu.h:42-55)."""
+    info.dev = DeviceBuffer.from_array(info.adj, _stream_of(stream))
+    return info
+
+
+@dataclass
+class BrickStorage:
+    """Device-resident BrickStorage: ``chunks`` x ``step`` elements."""
+
+    buf: DeviceBuffer
+    chunks: int
+    step: int
+    dtype: int = BK_F64
+
+    @classmethod
+    def allocate(cls, chunks: int, step: int, dtype: int = BK_F64) -> "BrickStorage":
+        p = C.c_void_p()
+        check(load().bk_storage_alloc(int(dtype), C.c_long(int(chunks)), C.c_size_t(int(step)),
+                                      C.byref(p)))
+        buf = DeviceBuffer.__new__(DeviceBuffer)
+        es = DTYPE_SIZE[dtype]
+        buf.nbytes = ((int(chunks) * int(step) * es + 2047) // 2048) * 2048
+        buf.ptr = p.value or 0
+        return cls(buf, int(chunks), int(step), int(dtype))
+
+    @property
+    def dat(self) -> int:
+        return self.buf.ptr
+
+    def to_host(self, stream=None) -> np.ndarray:
+        """movBrickStorage(..., DeviceToHost) (brick-gpu.h:84-103)."""
+        n = self.chunks * self.step
+        return self.buf.download(DTYPE_NP[self.dtype], n, _stream_of(stream)).reshape(
+            self.chunks, self.step)
+
+
+def movBrickStorage(host: np.ndarray, dtype: int = BK_F64, stream=None) -> BrickStorage:
+    """Copy a host storage array (chunks, step) into a new device
+    BrickStorage (brick-gpu.h:84-103)."""
+    host = np.ascontiguousarray(host, dtype=DTYPE_NP[dtype])
+    chunks, step = host.shape
+    st = BrickStorage.allocate(chunks, step, dtype)
+    st.buf.upload(host.reshape(-1), _stream_of(stream))
+    return st
+
+
+@dataclass
+class Brick:
+    """Brick<Dim<8,8,8>,Dim<8,8>>(bInfo, bStorage, offset): one field of a
+    storage, ``offset`` elements into each chunk (brick.h:390-394)."""
+
+    info: BrickInfo
+    storage: BrickStorage
+    offset: int = 0
+
+    @property
+    def ptr(self) -> int:
+        return self.storage.dat + self.offset * DTYPE_SIZE[self.storage.dtype]
+
+    @property
+    def step(self) -> int:
+        return self.storage.step
+
+    @property
+    def dtype(self) -> int:
+        return self.storage.dtype
+
+    def bricks_host(self, stream=None) -> np.ndarray:
+        """(chunks, 512) view of this field's bricks, copied to the host."""
+        return self.storage.to_host(stream)[:, self.offset:self.offset + BRICK_ELEMS]
+
+
+def copyToBrick(dimlist, padding, ghost, arr: DeviceBuffer, grid: DeviceBuffer, brick: Brick,
+                stream=None) -> None:
+    """copyToBrick<3> on the device (bricksetup.h:172-200)."""
+    check(load().bk_copy_to_brick(brick.dtype, longs(dimlist), longs(padding), longs(ghost),
+                                  _vp(arr.ptr), _vp(grid.ptr), _vp(brick.ptr),
+                                  C.c_size_t(brick.step), _vp(_stream_of(stream))))
+
+
+def copyFromBrick(dimlist, padding, ghost, arr: DeviceBuffer, grid: DeviceBuffer, brick: Brick,
+                  stream=None) -> None:
+    """copyFromBrick<3> on the device (bricksetup.h:213-221)."""
+    check(load().bk_copy_from_brick(brick.dtype, longs(dimlist), longs(padding), longs(ghost),
+                                    _vp(arr.ptr), _vp(grid.ptr), _vp(brick.ptr),
+                                    C.c_size_t(brick.step), _vp(_stream_of(stream))))
+
+
+# ---------------------------------------------------------------------------
+# periodic brick grid with one ghost layer
+# ---------------------------------------------------------------------------
+class PeriodicGrid:
+    """n = 8*nb interior cells per axis plus an 8-cell (one-brick) ghost
+    zone: a (nb+2)^3 brick grid numbered by init_grid, its adjacency, the
+    interior brick list and the periodic ghost pairs, all in HBM."""
+
+    def __init__(self, nb, stream=None):
+        self.nb = tuple(int(x) for x in nb)
+        self.padded = tuple(x + 2 for x in self.nb)
+        self.grid, self.info = init_grid(self.padded)
+        movBrickInfo(self.info, stream)
+        n = C.c_uint32()
+        check(load().bk_interior_list(longs(self.nb), None, C.byref(n)))
+        self.interior = np.empty(n.value, dtype=np.uint32)
+        check(load().bk_interior_list(longs(self.nb), self.interior.ctypes.data_as(_lib.u32p),
+                                      C.byref(n)))
+        check(load().bk_ghost_pairs(longs(self.nb), None, C.byref(n)))
+        self.pairs = np.empty(2 * n.value, dtype=np.uint32)
+        check(load().bk_ghost_pairs(longs(self.nb), self.pairs.ctypes.data_as(_lib.u32p),
+                                    C.byref(n)))
+        self.d_interior = DeviceBuffer.from_array(self.interior, stream)
+        self.d_pairs = DeviceBuffer.from_array(self.pairs, stream) if n.value else None
+        self.d_grid = DeviceBuffer.from_array(self.grid.reshape(-1), stream)
+
+    @property
+    def nbricks(self) -> int:
+        return self.info.nbricks
+
+    @property
+    def npairs(self) -> int:
+        return self.pairs.size // 2
+
+    @property
+    def cells(self) -> int:
+        return 512 * self.interior.size
+
+    def field_storage(self, dtype: int = BK_F64, fields: int = 1) -> BrickStorage:
+        """A storage of ``fields`` interleaved fields per chunk (step
+        512*fields), like stencils/3axis.cpp:26-28 when fields == 2."""
+        return BrickStorage.allocate(self.nbricks, BRICK_ELEMS * fields, dtype)
+
+    def fill_uniform(self, brick: Brick, seed: int, lo: float = -1.0, hi: float = 1.0,
+                     stream=None) -> None:
+        check(load().bk_fill_uniform(brick.dtype, _vp(brick.ptr), C.c_size_t(brick.step),
+                                     longs(self.nb), C.c_uint64(seed), lo, hi,
+                                     _vp(_stream_of(stream))))
+
+    def ghost_refresh(self, brick: Brick, stream=None) -> None:
+        if not self.npairs:
+            return
+        check(load().bk_ghost_refresh(brick.dtype, _vp(brick.ptr), C.c_size_t(brick.step),
+                                      C.c_size_t(BRICK_ELEMS), _vp(self.d_pairs.ptr),
+                                      C.c_uint32(self.npairs), _vp(_stream_of(stream))))
+
+    def stats(self, brick: Brick, stream=None) -> np.ndarray:
+        out = np.zeros(3)
+        check(load().bk_field_stats(brick.dtype, _vp(brick.ptr), C.c_size_t(brick.step),
+                                    _vp(self.d_interior.ptr), C.c_uint32(self.interior.size),
+                                    C.c_size_t(BRICK_ELEMS), out.ctypes.data_as(_lib.dblp),
+                                    _vp(_stream_of(stream))))
+        return out
+
+    def compare(self, a: Brick, ref: Brick, floor: float = 1.0, atol: float = 0.0,
+                rtol: float = 0.0, stream=None) -> dict:
+        out = np.zeros(5)
+        check(load().bk_field_compare(a.dtype, _vp(a.ptr), C.c_size_t(a.step), ref.dtype,
+                                      _vp(ref.ptr), C.c_size_t(ref.step),
+                                      _vp(self.d_interior.ptr), C.c_uint32(self.interior.size),
+                                      C.c_size_t(BRICK_ELEMS), floor, atol, rtol,
+                                      out.ctypes.data_as(_lib.dblp), _vp(_stream_of(stream))))
+        n = self.cells
+        return {"max_abs": out[0], "max_rel": out[1],
+                "rms_rel": float(np.sqrt(out[2] / out[3])) if out[3] > 0 else 0.0,
+                "rms_abs": float(np.sqrt(out[2] / n)), "bad": int(out[4])}
+
+
+def _desc(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, coeff_keep: list,
+          blist: DeviceBuffer | None = None, nlist: int | None = None) -> bk_sweep_desc:
+    if out.dtype != inp.dtype:
+        raise ValueError("input and output fields must share a dtype")
+    coeff = np.ascontiguousarray(st.coeff, dtype=np.float64)
+    if coeff.size != _lib.load().bk_stencil_ncoeff(st.kind):
+        raise ValueError(f"stencil {st.kind} takes {_lib.load().bk_stencil_ncoeff(st.kind)} "
+                         f"coefficients, got {coeff.size}")
+    coeff_keep.append(coeff)
+    d = bk_sweep_desc()
+    d.stencil = st.kind
+    d.dtype = inp.dtype
+    d.d_adj = grid.info.dptr
+    d.d_list = (blist or grid.d_interior).ptr
+    d.nlist = grid.interior.size if nlist is None else nlist
+    d.d_in = inp.ptr
+    d.in_step = inp.step
+    d.d_out = out.ptr
+    d.out_step = out.step
+    d.h_coeff = coeff.ctypes.data_as(_lib.dblp)
+    d.theta = float(st.theta)
+    d.cond_mode = int(st.cond_mode)
+    return d
+
+
+def sweep(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, stream=None) -> None:
+    """One sweep over the interior bricks: out = stencil(in)."""
+    keep: list = []
+    d = _desc(st, out, inp, grid, keep)
+    check(load().bk_sweep(C.byref(d), _vp(_stream_of(stream))))
+
+
+def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
+           use_graph: bool = False, refresh: bool = True, stream=None) -> Brick:
+    """``nsweeps`` ping-pong sweeps a->b->a..., each preceded by the periodic
+    ghost refresh of its input.  Returns the field holding the result."""
+    keep: list = []
+    d = _desc(st, b, a, grid, keep)
+    pairs = grid.d_pairs.ptr if (refresh and grid.d_pairs is not None) else None
+    npairs = grid.npairs if refresh else 0
+    check(load().bk_sweeps(C.byref(d), _vp(pairs), C.c_uint32(npairs), int(nsweeps),
+                           int(bool(use_graph)), _vp(_stream_of(stream))))
+    return b if nsweeps % 2 == 1 else a
+
+
+__all__ = [
+    "DeviceBuffer", "Stream", "Event", "BrickInfo", "init_grid", "movBrickInfo",
+    "BrickStorage", "movBrickStorage", "Brick", "copyToBrick", "copyFromBrick",
+    "PeriodicGrid", "sweep", "sweeps", "device_count", "device_sync", "DTYPE_NP",
+    "DTYPE_SIZE", "BK_F64", "BK_F32", "BK_BF16",
+]

@@ -1,0 +1,61 @@
+// bk_internal.h -- host-side declarations shared by the library's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bk_api.h"
+
+namespace bk {
+
+// Normalised sweep description handed from the C-ABI to the launchers.
+struct SweepLaunch {
+  int stencil;
+  int dtype;
+  const uint32_t *adj;
+  const uint32_t *list;
+  uint32_t nlist;
+  const void *in;
+  size_t in_step;
+  void *out;
+  size_t out_step;
+  double coeff[125];
+  int ncoeff;
+  double theta;
+  int cond_mode;
+};
+
+// Kernel variants (performance options, results identical).
+enum StarVariant { STAR_LDS = 0 };
+enum CubeVariant { CUBE_LDS = 0 };
+
+hipError_t launch_star(const SweepLaunch &L, hipStream_t s);
+hipError_t launch_cube(const SweepLaunch &L, hipStream_t s);
+
+hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,
+                                size_t brick_bytes, const uint32_t *pairs,
+                                uint32_t npairs, hipStream_t s);
+hipError_t launch_zero(void *dst, size_t bytes, hipStream_t s);
+hipError_t launch_fill_uniform(int dtype, void *dat, size_t step,
+                               const long nb[3], uint64_t seed, double lo,
+                               double hi, hipStream_t s);
+// Deterministic two-pass FP64 reductions; `scratch` holds kReduceBlocks*8
+// doubles.  Results land in scratch[0 .. nout).
+constexpr int kReduceBlocks = 1024;
+hipError_t launch_field_stats(int dtype, const void *dat, size_t step,
+                              const uint32_t *list, uint32_t nlist,
+                              size_t brick_elems, double *scratch,
+                              hipStream_t s);
+hipError_t launch_field_compare(int dta, const void *a, size_t step_a,
+                                int dtr, const void *r, size_t step_r,
+                                const uint32_t *list, uint32_t nlist,
+                                size_t brick_elems, double floor_,
+                                double atol, double rtol, double *scratch,
+                                hipStream_t s);
+hipError_t launch_copy_brick(int to_brick, int dtype, const long dims[3],
+                             const long padding[3], const long ghost[3],
+                             void *arr, const uint32_t *grid, void *dat,
+                             size_t step, hipStream_t s);
+
+}  // namespace bk

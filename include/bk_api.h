@@ -1,0 +1,225 @@
+/*
+ * bk_api.h -- C-ABI of the MI355X brick-stencil library (libbricks.so).
+ *
+ * This is the drop-in boundary.  bricklib itself is header-only C++ with no
+ * foreign-function interface; its GPU work is reached through the template
+ * headers (brick.h, bricksetup.h, brick-gpu.h) and kernels written against
+ * them.  Every entry point below names the reference interface it stands in
+ * for (file:line in CtopCsUtahEdu/bricklib).  The C++ headers shipped beside
+ * this file (brick.h, bricksetup.h, brickcompare.h, brick-hip.h) keep the
+ * reference's template API and forward device work here.
+ *
+ * Conventions
+ *   - Every function returns an int status (BK_OK == 0).  Nothing throws or
+ *     exits across this boundary; bk_last_error() gives a message for the
+ *     calling thread's last failure.
+ *   - Pointers named d_* are device (HBM) pointers, h_* are host pointers.
+ *   - Brick data: element e (0 <= e < brick_elems) of brick b of a field is
+ *     at  field[(size_t)b * step + e]  where `field` is the storage base
+ *     already offset to that field (BrickStorage + offset, brick.h:390-394).
+ *   - Adjacency: uint32 adj[nbricks][27]; slot kd*9 + jd*3 + id with
+ *     d = offset + 1 in {0,1,2}; slot 13 is the brick itself
+ *     (brick.h:96-127, accessor brick.h:235-246).
+ *   - All launches are stream-ordered on the stream argument (NULL = the
+ *     device's null stream).  No entry point that launches kernels
+ *     allocates device memory, except bk_sweeps with use_graph != 0, which
+ *     builds and caches an executable graph on first use.
+ */
+#ifndef BK_API_H
+#define BK_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BK_API_VERSION 1
+
+enum bk_status {
+  BK_OK = 0,
+  BK_ERR_ARG = 1,         /* invalid argument (null pointer, bad size, bad enum) */
+  BK_ERR_NOMEM = 2,       /* device or host allocation failed */
+  BK_ERR_HIP = 3,         /* HIP runtime error (message in bk_last_error) */
+  BK_ERR_UNSUPPORTED = 4, /* combination not built (e.g. BF16 cube) */
+  BK_ERR_NODEV = 5        /* no HIP device visible */
+};
+
+/* Element type of a brick field.  bElem (vecscatter.h:12-14) is double in
+ * the reference; FP32 and BF16 storage are additions. */
+enum bk_dtype { BK_F64 = 0, BK_F32 = 1, BK_BF16 = 2 };
+
+/* Stencil kinds.  Coefficient layouts (ncoeff values, double):
+ *   STAR7/13/25 (radius R = 1/2/4): c[0] centre, then for r = 1..R the six
+ *     taps (i+r, i-r, j+r, j-r, k+r, k-r) at c[6(r-1)+1 .. 6(r-1)+6]
+ *     -- the order of stencils/7pt.py:19-26, mpi13pt.py:18-31,
+ *     mpi25pt.py:20-45 (7pt: coeff[1] is i+1 ... coeff[6] is k-1).
+ *   CUBE125 (radius 2): c[(dk+2)*25 + (dj+2)*5 + (di+2)], 125 values
+ *     (the 5x5x5 offsets of mpi125pt.py:12-162 with one coefficient each).
+ *   COND7: the 7 STAR7 coefficients; p = sum c_n * max(x_n, 0) and
+ *     out = (s > t) ? p : -p, where (s, t) = (p, 0) in the reference form
+ *     (cond.py:23-32, 3axis.cpp:132-137) and (centre, theta) in the
+ *     threshold form.                                                    */
+enum bk_stencil {
+  BK_STAR7 = 0,
+  BK_STAR13 = 1,
+  BK_STAR25 = 2,
+  BK_CUBE125 = 3,
+  BK_COND7 = 4
+};
+
+enum bk_cond_mode { BK_COND_REFERENCE = 0, BK_COND_THRESHOLD = 1 };
+
+typedef void *bk_stream; /* hipStream_t */
+typedef void *bk_event;  /* hipEvent_t */
+
+/* ---- library, errors, device ------------------------------------------ */
+int bk_version(void);
+const char *bk_last_error(void);
+/* Number of coefficients a stencil takes (0 for an unknown kind). */
+int bk_stencil_ncoeff(int stencil);
+/* Radius of a stencil (max |offset| along an axis), 0 for an unknown kind. */
+int bk_stencil_radius(int stencil);
+int bk_dtype_size(int dtype);
+int bk_device_count(int *n);
+int bk_set_device(int dev);
+int bk_device_sync(void);
+
+/* ---- memory: device-resident BrickStorage ----------------------------- */
+/* BrickStorage::allocate(chunks, step) (brick.h:68-75) with the data in HBM:
+ * chunks * step elements of `dtype`, 2048-byte aligned like the reference's
+ * ALIGN (brick.h:15).  Fails with BK_ERR_ARG when the byte count overflows. */
+int bk_storage_alloc(int dtype, long chunks, size_t step, void **d_dat);
+int bk_free(void *d_ptr);
+int bk_alloc(size_t bytes, void **d_ptr);
+/* Host<->device copies: movBrickStorage (brick-gpu.h:84-103) and
+ * movBrickInfo (brick-gpu.h:42-55) are built on these.  Sizes are size_t
+ * (the reference computes them in `unsigned`, brick-gpu.h:47,89). */
+int bk_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes, bk_stream s);
+int bk_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes, bk_stream s);
+int bk_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, bk_stream s);
+/* Zero-fill with a kernel (no hipMemset node, so it is safe in graphs). */
+int bk_zero(void *d_dst, size_t bytes, bk_stream s);
+
+/* ---- streams and events (hipEvent timing, stencils.hip.h:13-31) ------- */
+int bk_stream_create(bk_stream *s);
+int bk_stream_destroy(bk_stream s);
+int bk_stream_sync(bk_stream s);
+int bk_event_create(bk_event *e);
+int bk_event_destroy(bk_event e);
+int bk_event_record(bk_event e, bk_stream s);
+int bk_event_sync(bk_event e);
+int bk_event_elapsed_ms(bk_event start, bk_event stop, float *ms);
+
+/* ---- grid numbering and adjacency (host) ------------------------------- */
+/* init_grid<3> (bricksetup.h:73-89): bricks of a dims[0] x dims[1] x
+ * dims[2] brick grid (i fastest, dims contiguous-first) numbered
+ * grid[pos] = pos, and adj[pos][27] filled from that numbering.  With
+ * periodic == 0 a neighbour outside the grid is brick 0, exactly as the
+ * reference (bricksetup.h:42-47, which also wraps edge rows); with
+ * periodic != 0 every neighbour coordinate is taken modulo dims.
+ * h_grid: dims[0]*dims[1]*dims[2] uint32, h_adj: 27 * that many uint32. */
+int bk_init_grid(const long dims[3], int periodic, uint32_t *h_grid,
+                 uint32_t *h_adj);
+/* Periodic ghost-brick pairs for a grid of nb[0] x nb[1] x nb[2] interior
+ * bricks surrounded by one layer of ghost bricks ((nb+2)^3 numbered as
+ * bk_init_grid).  Writes (dst, src) pairs: every ghost brick and the
+ * interior brick that is its periodic image (the one-rank periodic exchange
+ * of BrickDecomp::exchange, brick-mpi.h:466-495, over the periodic
+ * Cartesian communicator of weak/args.cpp:107-108).  h_pairs may be NULL to
+ * query *npairs. */
+int bk_ghost_pairs(const long nb[3], uint32_t *h_pairs, uint32_t *npairs);
+/* Interior brick list in grid order (k slowest, i fastest) for the same
+ * padded grid: the bricks the sweep loops of stencils/3axis.cpp:51-66 visit.
+ * h_list may be NULL to query *n. */
+int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n);
+
+/* ---- the sweep -------------------------------------------------------- */
+typedef struct bk_sweep_desc {
+  int stencil;            /* enum bk_stencil */
+  int dtype;              /* enum bk_dtype of both fields */
+  const uint32_t *d_adj;  /* [nbricks_total][27] adjacency in HBM */
+  const uint32_t *d_list; /* brick indices to compute, in HBM */
+  uint32_t nlist;         /* entries in d_list */
+  const void *d_in;       /* input field base (already offset) */
+  size_t in_step;         /* elements between consecutive bricks of d_in */
+  void *d_out;            /* output field base (already offset) */
+  size_t out_step;
+  const double *h_coeff;  /* host coefficients, bk_stencil_ncoeff values */
+  double theta;           /* COND7 threshold (threshold form) */
+  int cond_mode;          /* enum bk_cond_mode */
+} bk_sweep_desc;
+
+/* One stencil sweep: for every brick b in d_list and every element of b,
+ * out[b] = stencil(in) read through the adjacency.  This is the loop of
+ * stencils/3axis.cpp:51-66 (7pt), :123-140 (cond), and the brick(...)
+ * kernel of weak/main.hip.cpp:32-40 (13/25/125-point via
+ * stencils/mpi*.py), for 8x8x8 bricks in the 8x8 vector fold.
+ * Accumulation order per point (fixed, so results are reproducible):
+ * acc = c_t0 * x_t0, then acc = fma(c_t, x_t, acc) over the remaining taps,
+ * in the coefficient order above for stars and (dk, dj, di) lexicographic
+ * for the cube.  FP64 accumulates in FP64; FP32 and BF16 in FP32. */
+int bk_sweep(const bk_sweep_desc *d, bk_stream s);
+
+/* Periodic ghost refresh on the device: for each (dst, src) pair, copy the
+ * brick_elems elements of brick src onto brick dst in the field d_dat.
+ * Replaces the MPI exchange of weak/main.hip.cpp:249-269 for one rank. */
+int bk_ghost_refresh(int dtype, void *d_dat, size_t step, size_t brick_elems,
+                     const uint32_t *d_pairs, uint32_t npairs, bk_stream s);
+
+/* nsweeps ping-pong sweeps with a periodic ghost refresh of the current
+ * input before each sweep (weak/main.hip.cpp:274-279 with the exchange of
+ * :249-269 done on the device).  Sweep 0 reads d_in and writes d_out,
+ * sweep 1 reads d_out and writes d_in, ...; the result is in d_out when
+ * nsweeps is odd and in d_in when it is even.  With use_graph != 0 the
+ * whole sequence is captured once into a hipGraph (cached per argument set)
+ * and replayed. */
+int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs,
+              int nsweeps, int use_graph, bk_stream s);
+/* Drop every cached graph (frees their device resources). */
+int bk_graph_cache_clear(void);
+
+/* ---- seeded input and FP64 reductions ---------------------------------- */
+/* Fill the interior of a padded brick grid (nb interior bricks per axis,
+ * one ghost layer) with the seeded field value(x,y,z) = lo + (hi-lo) * u,
+ * u = the 53-bit uniform of hash(seed, x + n0*(y + n1*z)) (n = 8*nb),
+ * converted to dtype with round-to-nearest-even.  Ghost bricks untouched. */
+int bk_fill_uniform(int dtype, void *d_dat, size_t step, const long nb[3],
+                    uint64_t seed, double lo, double hi, bk_stream s);
+/* FP64 reductions over the bricks in d_list (brick_elems each):
+ *   out[0] = sum x, out[1] = sum x^2, out[2] = max |x|
+ * written to the host array h_out after a stream sync. */
+int bk_field_stats(int dtype, const void *d_dat, size_t step,
+                   const uint32_t *d_list, uint32_t nlist, size_t brick_elems,
+                   double h_out[3], bk_stream s);
+/* FP64 comparison of field a against reference field r (dtypes may differ)
+ * over d_list:  out[0] = max |a-r|,  out[1] = max |a-r| / max(|r|, floor),
+ * out[2] = sum (a-r)^2,  out[3] = sum r^2,
+ * out[4] = count of points with |a-r| > atol + rtol * |r| (compareBrick's
+ * rule of brickcompare.h:35-39 with separate tolerances).  */
+int bk_field_compare(int dtype_a, const void *d_a, size_t step_a, int dtype_r,
+                     const void *d_r, size_t step_r, const uint32_t *d_list,
+                     uint32_t nlist, size_t brick_elems, double floor,
+                     double atol, double rtol, double h_out[5], bk_stream s);
+
+/* ---- array <-> brick conversion on the device -------------------------- */
+/* copyToBrick / copyFromBrick (bricksetup.h:172-221) on the device for
+ * 8x8x8 bricks: the array is contiguous-first with extents
+ * dims[a] + 2*(padding[a] + ghost[a]); the bricks of d_grid (a uint32 grid
+ * of (dims[a] + 2*ghost[a]) / 8 bricks per axis) cover dims + 2*ghost.
+ * Array and bricks share `dtype`. */
+int bk_copy_to_brick(int dtype, const long dims[3], const long padding[3],
+                     const long ghost[3], const void *d_arr,
+                     const uint32_t *d_grid, void *d_dat, size_t step,
+                     bk_stream s);
+int bk_copy_from_brick(int dtype, const long dims[3], const long padding[3],
+                       const long ghost[3], void *d_arr,
+                       const uint32_t *d_grid, const void *d_dat, size_t step,
+                       bk_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BK_API_H */

@@ -1,0 +1,110 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol that
+include/bk_api.h declares, its host-side helpers agree with the oracle, and
+argument errors come back as status codes (checked before any HIP call)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        if 'extern "C"' not in txt:
+            continue
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(bk_[a-z_0-9]+)\s*\(", txt,
+                            flags=re.M)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol(bk):
+    lib = bk.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_python_binding_covers_header(bk):
+    from bricklib_amd._lib import SIGNATURES
+    assert set(declared_functions()) <= set(SIGNATURES)
+
+
+def test_stencil_tables(bk):
+    lib = bk.load()
+    for st in range(5):
+        assert lib.bk_stencil_ncoeff(st) == O.lib().or_stencil_ncoeff(st)
+        assert lib.bk_stencil_radius(st) == O.lib().or_stencil_radius(st)
+    assert lib.bk_stencil_ncoeff(99) == 0
+    assert [lib.bk_dtype_size(d) for d in (0, 1, 2, 7)] == [8, 4, 2, 0]
+
+
+@pytest.mark.parametrize("nb", [(1, 1, 1), (2, 3, 4), (7, 1, 2)])
+def test_grid_helpers_match_oracle(bk, nb):
+    from bricklib_amd._lib import longs, u32p
+    lib = bk.load()
+    padded = [x + 2 for x in nb]
+    g_or, a_or = O.init_grid(padded)
+    g, info = bk.init_grid(padded)
+    np.testing.assert_array_equal(g.reshape(-1), g_or)
+    np.testing.assert_array_equal(info.adj, a_or)
+    gp, ap = O.init_grid(padded, periodic=True)
+    g2, info2 = bk.init_grid(padded, periodic=True)
+    np.testing.assert_array_equal(info2.adj, ap)
+    n = C.c_uint32()
+    assert lib.bk_interior_list(longs(nb), None, C.byref(n)) == 0
+    lst = np.empty(n.value, np.uint32)
+    lib.bk_interior_list(longs(nb), lst.ctypes.data_as(u32p), C.byref(n))
+    np.testing.assert_array_equal(lst, O.interior_list(nb))
+    assert lib.bk_ghost_pairs(longs(nb), None, C.byref(n)) == 0
+    pairs = np.empty(2 * n.value, np.uint32)
+    lib.bk_ghost_pairs(longs(nb), pairs.ctypes.data_as(u32p), C.byref(n))
+    np.testing.assert_array_equal(pairs, O.ghost_pairs(nb))
+    total = int(np.prod(padded))
+    assert n.value == total - int(np.prod(nb))
+
+
+def test_argument_errors_without_gpu(bk):
+    from bricklib_amd._lib import BK_ERR_ARG, bk_sweep_desc, longs
+    lib = bk.load()
+    assert lib.bk_sweep(None, None) == BK_ERR_ARG
+    assert b"null descriptor" in lib.bk_last_error()
+    d = bk_sweep_desc()
+    d.stencil = 42
+    assert lib.bk_sweep(C.byref(d), None) == BK_ERR_ARG
+    d.stencil = 0
+    d.dtype = 9
+    assert lib.bk_sweep(C.byref(d), None) == BK_ERR_ARG
+    d.dtype = 0
+    coeff = np.ones(7)
+    d.h_coeff = coeff.ctypes.data_as(C.POINTER(C.c_double))
+    d.nlist = 4
+    d.d_adj = d.d_list = d.d_in = d.d_out = 4096
+    d.in_step = d.out_step = 512
+    assert lib.bk_sweep(C.byref(d), None) == BK_ERR_ARG  # in == out aliasing
+    assert b"alias" in lib.bk_last_error()
+    d.d_out = 8192
+    d.in_step = 100
+    assert lib.bk_sweep(C.byref(d), None) == BK_ERR_ARG
+    p = C.c_void_p()
+    assert lib.bk_storage_alloc(5, 10, 512, C.byref(p)) == BK_ERR_ARG
+    assert lib.bk_storage_alloc(0, -1, 512, C.byref(p)) == BK_ERR_ARG
+    assert lib.bk_storage_alloc(0, 1 << 62, 1 << 20, C.byref(p)) == BK_ERR_ARG
+    assert lib.bk_init_grid(longs([0, 1, 1]), 0, None, None) == BK_ERR_ARG
+    assert lib.bk_sweeps(C.byref(d), None, 0, -1, 0, None) == BK_ERR_ARG
+
+
+def test_python_mirror_raises(bk):
+    from bricklib_amd import BrickError
+    with pytest.raises(BrickError):
+        bk.BrickStorage.allocate(-1, 512)

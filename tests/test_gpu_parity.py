@@ -1,0 +1,226 @@
+"""GPU parity: the HIP sweeps (through the C-ABI) against the oracle.
+
+Bar: bit-exact against the oracle's fixed-order (FMA) evaluation for every
+stencil and dtype; FP64 within 1e-12 (one sweep) / 1e-10 (100 sweeps) of
+the reference-order evaluation, relative error per point with the floor
+max(|ref|, 1) -- the inputs are U[-1,1], so the floor is the input scale
+and matches the absolute branch of compareBrick (brickcompare.h:35-39).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from bricklib_amd import stencils as S
+
+pytestmark = pytest.mark.gpu
+
+STENCILS = [S.BK_STAR7, S.BK_STAR13, S.BK_STAR25, S.BK_CUBE125, S.BK_COND7]
+DTYPES = [0, 1, 2]  # F64, F32, BF16
+NAMES = {0: "7pt", 1: "13pt", 2: "25pt", 3: "125pt", 4: "cond"}
+
+
+def setup(bk, nb, dtype, seed, fields=1):
+    g = bk.PeriodicGrid(nb)
+    st = g.field_storage(dtype, fields)
+    a = bk.Brick(g.info, st, 0)
+    sb = g.field_storage(dtype, fields)
+    b = bk.Brick(g.info, sb, 0)
+    g.fill_uniform(a, seed)
+    sb.buf.zero()
+    og = O.Grid(nb)
+    oa = og.storage(dtype, fields)
+    og.fill_uniform(oa, seed, dtype=dtype)
+    return g, a, b, og, oa
+
+
+def host(brick):
+    return brick.storage.to_host()
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_fill_matches_oracle(gpu, dtype):
+    g, a, b, og, oa = setup(gpu, (2, 3, 2), dtype, 17)
+    np.testing.assert_array_equal(host(a)[og.interior], oa[og.interior])
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_ghost_refresh_matches_oracle(gpu, dtype):
+    g, a, b, og, oa = setup(gpu, (3, 2, 2), dtype, 5)
+    g.ghost_refresh(a)
+    og.refresh(oa, dtype)
+    np.testing.assert_array_equal(host(a), oa)
+
+
+@pytest.mark.parametrize("stencil", STENCILS, ids=lambda s: NAMES[s])
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32", "bf16"])
+def test_one_sweep_bit_exact(gpu, stencil, dtype):
+    print(f"[case] one sweep {NAMES[stencil]} dtype={dtype} nb=(2,3,4)", flush=True)
+    st = S.Stencil.make(stencil, seed=7)
+    g, a, b, og, oa = setup(gpu, (2, 3, 4), dtype, 3)
+    g.ghost_refresh(a)
+    gpu.sweep(st, b, a, g)
+    og.refresh(oa, dtype)
+    ob = og.storage(dtype)
+    og.sweep(stencil, st.coeff, oa, ob, dtype=dtype, theta=st.theta, cond_mode=st.cond_mode)
+    got = host(b)
+    np.testing.assert_array_equal(got[og.interior], ob[og.interior])
+
+
+@pytest.mark.parametrize("stencil", STENCILS, ids=lambda s: NAMES[s])
+def test_fp64_within_1e12_of_reference_order(gpu, stencil):
+    st = S.Stencil.make(stencil, seed=2)
+    g, a, b, og, oa = setup(gpu, (3, 3, 3), 0, 8)
+    g.ghost_refresh(a)
+    gpu.sweep(st, b, a, g)
+    og.refresh(oa)
+    ob = og.storage()
+    og.sweep(stencil, st.coeff, oa, ob, mode=O.MODE_REF, theta=st.theta, cond_mode=st.cond_mode)
+    err = O.field_compare(host(b), ob, og.interior, floor=1.0)
+    print(f"[err] {NAMES[stencil]} 1 sweep vs reference order: {err}")
+    assert err["max_rel"] <= 1e-12
+
+
+@pytest.mark.parametrize("stencil", STENCILS, ids=lambda s: NAMES[s])
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+def test_pingpong_sweeps_bit_exact(gpu, stencil, use_graph):
+    print(f"[case] 8 sweeps {NAMES[stencil]} graph={use_graph}", flush=True)
+    st = S.Stencil.make(stencil, seed=4)
+    g, a, b, og, oa = setup(gpu, (2, 2, 3), 0, 12)
+    res = gpu.sweeps(st, a, b, g, 8, use_graph=use_graph)
+    ob = og.storage()
+    ores = og.sweeps(stencil, st.coeff, oa, ob, 8, theta=st.theta, cond_mode=st.cond_mode)
+    np.testing.assert_array_equal(host(res)[og.interior], ores[og.interior])
+
+
+def test_graph_replays_identical(gpu):
+    st = S.Stencil.make("7pt", seed=1)
+    g, a, b, og, oa = setup(gpu, (2, 2, 2), 0, 1)
+    ref = None
+    for rep in range(4):
+        g.fill_uniform(a, 1)
+        gpu.sweeps(st, a, b, g, 4, use_graph=True)
+        gpu.device_sync()
+        cur = host(a)[og.interior]  # 4 sweeps: result back in a
+        if ref is None:
+            ref = cur
+            g.fill_uniform(a, 1)
+            gpu.sweeps(st, a, b, g, 4, use_graph=False)
+            np.testing.assert_array_equal(host(a)[og.interior], ref)
+        else:
+            np.testing.assert_array_equal(cur, ref)
+
+
+@pytest.mark.parametrize("stencil", [S.BK_STAR7, S.BK_CUBE125], ids=["7pt", "125pt"])
+def test_interleaved_fields(gpu, stencil):
+    """bIn at offset 0 and bOut at offset 512 of one storage with step 1024,
+    as stencils/3axis.cpp:26-28 lays them out."""
+    st = S.Stencil.make(stencil, seed=9)
+    g = gpu.PeriodicGrid((2, 2, 2))
+    storage = g.field_storage(0, fields=2)
+    storage.buf.zero()
+    bin_, bout = gpu.Brick(g.info, storage, 0), gpu.Brick(g.info, storage, 512)
+    g.fill_uniform(bin_, 31)
+    g.ghost_refresh(bin_)
+    gpu.sweep(st, bout, bin_, g)
+    og = O.Grid((2, 2, 2))
+    oa = og.storage()
+    og.fill_uniform(oa, 31)
+    og.refresh(oa)
+    ob = og.storage()
+    og.sweep(stencil, st.coeff, oa, ob)
+    h = storage.to_host()
+    np.testing.assert_array_equal(h[og.interior, 512:], ob[og.interior])
+    np.testing.assert_array_equal(h[og.interior, :512], oa[og.interior])
+
+
+@pytest.mark.parametrize("nb", [(1, 1, 1), (1, 2, 5), (7, 3, 1)])
+def test_ragged_and_single_brick_grids(gpu, nb):
+    for stencil in STENCILS:
+        st = S.Stencil.make(stencil, seed=6)
+        g, a, b, og, oa = setup(gpu, nb, 0, 2)
+        res = gpu.sweeps(st, a, b, g, 3)
+        ob = og.storage()
+        ores = og.sweeps(stencil, st.coeff, oa, ob, 3, theta=st.theta, cond_mode=st.cond_mode)
+        np.testing.assert_array_equal(host(res)[og.interior], ores[og.interior])
+
+
+def test_plain_case_128(gpu):
+    """Plain method at 128^3: 7-point star, FP64, coefficients U(0,1)
+    normalised, inputs U[-1,1], periodic ghost bricks; 1 sweep, then 8
+    ping-pong sweeps; against the oracle's brick loop (bit-exact) and its
+    array loop in the reference order (<= 1e-12)."""
+    nb = (16, 16, 16)
+    coeff = S.coeff_plain7(2024)
+    st = S.Stencil(S.BK_STAR7, coeff)
+    g, a, b, og, oa = setup(gpu, nb, 0, 99)
+    g.ghost_refresh(a)
+    gpu.sweep(st, b, a, g)
+    og.refresh(oa)
+    ob = og.storage()
+    og.sweep(0, coeff, oa, ob)
+    np.testing.assert_array_equal(host(b)[og.interior], ob[og.interior])
+    obr = og.storage()
+    og.sweep(0, coeff, oa, obr, mode=O.MODE_REF)
+    e1 = O.field_compare(host(b), obr, og.interior, floor=1.0)
+    assert e1["max_rel"] <= 1e-12
+    # 8 ping-pong sweeps from the same input
+    g.fill_uniform(a, 99)
+    res = gpu.sweeps(st, a, b, g, 8)
+    oa2 = og.storage()
+    og.fill_uniform(oa2, 99)
+    ob2 = og.storage()
+    ores = og.sweeps(0, coeff, oa2, ob2, 8)
+    np.testing.assert_array_equal(host(res)[og.interior], ores[og.interior])
+    oa3 = og.storage()
+    og.fill_uniform(oa3, 99)
+    ob3 = og.storage()
+    oref = og.sweeps(0, coeff, oa3, ob3, 8, mode=O.MODE_REF)
+    e8 = O.field_compare(host(res), oref, og.interior, floor=1.0)
+    print(f"[err] plain 128^3: 1 sweep {e1['max_rel']:.3e}, 8 sweeps {e8['max_rel']:.3e}")
+    assert e8["max_rel"] <= 1e-10
+
+
+def test_device_reductions(gpu):
+    g, a, b, og, oa = setup(gpu, (3, 2, 2), 0, 42)
+    s = g.stats(a)
+    x = oa[og.interior].reshape(-1)
+    assert abs(s[0] - x.sum()) < 1e-9
+    assert abs(s[1] - (x * x).sum()) < 1e-9
+    assert s[2] == np.abs(x).max()
+    g2 = g.stats(a)
+    np.testing.assert_array_equal(s, g2)  # deterministic
+    c = g.compare(a, a)
+    assert c["max_abs"] == 0 and c["bad"] == 0
+
+
+def test_device_copy_to_from_brick(gpu):
+    dims, pad, ghost = [16, 24, 8], [8, 8, 8], [8, 8, 8]
+    bdims = [(d + 2 * gh) // 8 for d, gh in zip(dims, ghost)]
+    grid, info = gpu.init_grid(bdims)
+    ext = [d + 2 * (p + gh) for d, p, gh in zip(dims, pad, ghost)]
+    arr = np.random.default_rng(0).standard_normal(int(np.prod(ext)))
+    d_arr = gpu.DeviceBuffer.from_array(arr)
+    d_grid = gpu.DeviceBuffer.from_array(grid.reshape(-1))
+    st = gpu.BrickStorage.allocate(info.nbricks, 512)
+    st.buf.zero()
+    br = gpu.Brick(info, st, 0)
+    gpu.copyToBrick(dims, pad, ghost, d_arr, d_grid, br)
+    exp = np.zeros((info.nbricks, 512))
+    O.lib().or_copy_to_brick(0, O.L3(dims), O.L3(pad), O.L3(ghost), O.P(arr),
+                             O.P(grid.reshape(-1), O.u32p), O.P(exp), 512)
+    np.testing.assert_array_equal(st.to_host(), exp)
+    out = gpu.DeviceBuffer(arr.nbytes)
+    out.zero()
+    gpu.copyFromBrick(dims, pad, ghost, out, d_grid, br)
+    back = out.download(np.float64)
+    expb = np.zeros_like(arr)
+    O.lib().or_copy_from_brick(0, O.L3(dims), O.L3(pad), O.L3(ghost), O.P(expb),
+                               O.P(grid.reshape(-1), O.u32p), O.P(exp), 512)
+    np.testing.assert_array_equal(back, expb)
+
+
+def test_aliasing_rejected(gpu):
+    st = S.Stencil.make("7pt")
+    g, a, b, og, oa = setup(gpu, (1, 1, 1), 0, 1)
+    with pytest.raises(gpu.BrickError):
+        gpu.sweep(st, a, a, g)
