@@ -120,7 +120,7 @@ def cpu_baseline(nb, coeff, seconds):
         src, dst = dst, src
         n += 1
         el = time.perf_counter() - t0
-        if el >= seconds or n >= 50:
+        if el >= seconds or n >= 1000:
             break
     pts = g.interior.size * 512
     return {"value": pts * n / el / 1e9, "unit": "GStencil/s",

@@ -9,14 +9,15 @@ first call that needs it does, and fails loudly when it is not built.
 from ._lib import (BK_BF16, BK_COND7, BK_COND_REFERENCE, BK_COND_THRESHOLD, BK_CUBE125,
                    BK_F32, BK_F64, BK_STAR7, BK_STAR13, BK_STAR25, BrickError, load)
 from .bricks import (Brick, BrickInfo, BrickStorage, DeviceBuffer, Event, PeriodicGrid,
-                     Stream, copyFromBrick, copyToBrick, device_count, device_sync, init_grid,
+                     Plan, Stream, plan_host, copyFromBrick, copyToBrick, device_count, device_sync, init_grid,
                      movBrickInfo, movBrickStorage, sweep, sweeps)
 from .stencils import Stencil
 
 __all__ = [
     "BK_F64", "BK_F32", "BK_BF16", "BK_STAR7", "BK_STAR13", "BK_STAR25", "BK_CUBE125",
     "BK_COND7", "BK_COND_REFERENCE", "BK_COND_THRESHOLD", "BrickError", "load", "Brick",
-    "BrickInfo", "BrickStorage", "DeviceBuffer", "Event", "PeriodicGrid", "Stream",
+    "BrickInfo", "BrickStorage", "DeviceBuffer", "Event", "PeriodicGrid", "Plan", "Stream",
+    "plan_host",
     "copyFromBrick", "copyToBrick", "device_count", "device_sync", "init_grid", "movBrickInfo",
     "movBrickStorage", "sweep", "sweeps", "Stencil",
 ]

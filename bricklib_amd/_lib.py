@@ -20,6 +20,7 @@ BK_ERR_NODEV = 5
 BK_F64, BK_F32, BK_BF16 = 0, 1, 2
 BK_STAR7, BK_STAR13, BK_STAR25, BK_CUBE125, BK_COND7 = 0, 1, 2, 3, 4
 BK_COND_REFERENCE, BK_COND_THRESHOLD = 0, 1
+BK_PLAN_COLUMN, BK_PLAN_QUAD = 1, 4
 
 u32p = C.POINTER(C.c_uint32)
 dblp = C.POINTER(C.c_double)
@@ -41,6 +42,7 @@ class bk_sweep_desc(C.Structure):
         ("h_coeff", dblp),
         ("theta", C.c_double),
         ("cond_mode", C.c_int),
+        ("plan", vp),
     ]
 
 
@@ -82,6 +84,13 @@ SIGNATURES = {
     "bk_field_compare": (C.c_int, [C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_size_t, vp,
                                    C.c_uint32, C.c_size_t, C.c_double, C.c_double, C.c_double,
                                    dblp, vp]),
+    "bk_plan_create": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, C.c_int, C.c_uint32,
+                                 C.POINTER(vp)]),
+    "bk_plan_destroy": (C.c_int, [vp]),
+    "bk_plan_query": (C.c_int, [vp, u32p, u32p, C.POINTER(C.c_int), u32p]),
+    "bk_plan_download": (C.c_int, [vp, u32p, u32p, u32p, u32p]),
+    "bk_plan_build_host": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, C.c_int, C.c_uint32,
+                                     u32p, u32p, u32p, u32p, u32p, u32p]),
     "bk_copy_to_brick": (C.c_int, [C.c_int, longp, longp, longp, vp, vp, vp, C.c_size_t, vp]),
     "bk_copy_from_brick": (C.c_int, [C.c_int, longp, longp, longp, vp, vp, vp, C.c_size_t, vp]),
 }

@@ -12,11 +12,16 @@
 // plane within reach, so each LDS value serves up to five FMAs.  Per point
 // the taps accumulate in (dk, dj, di) lexicographic order, starting with a
 // multiply -- the coefficient order of the API.
+#include <cstddef>
+
 #include "bk_device.h"
 #include "bk_internal.h"
 
 namespace bk {
 namespace {
+
+template <class A>
+using kconst_ptr = const A __attribute__((address_space(4))) *;
 
 template <class T>
 struct CubeArgs {
@@ -28,6 +33,12 @@ struct CubeArgs {
   uint64_t in_step;
   uint64_t out_step;
   uint32_t nlist;
+  // plan (quad kernel only)
+  const uint32_t *__restrict__ anchor;
+  const uint32_t *__restrict__ len;
+  const uint32_t *__restrict__ mask;
+  uint32_t nunits;
+  uint32_t kcol;
   A c[125];
 };
 
@@ -90,6 +101,210 @@ __global__ __launch_bounds__(256) void cube_sweep_lds(const CubeArgs<T> a) {
   for (int k = 0; k < 8; ++k) narrow(po[k * 64 + lane], acc[k]);
 }
 
+
+// Plane-streaming variant over 2x2 column quads (plan shape BK_PLAN_QUAD).
+//
+// One wave per plan unit: a 2x2 block of brick columns (members m0, m1 =
+// i+1, m2 = j+1, m3 = i+1,j+1) over up to kcol levels.  The wave streams the
+// input planes P = -2 .. 8L+1 of the 16x16 quad plane, each padded by the
+// 2-deep halo to 20x20 and gathered through the adjacency into one of two
+// LDS buffers (the next plane loads while the current one computes).  Lane
+// (lj, li) owns the 2x2 points (2lj+pj, 2li+pi) of the quad plane and keeps
+// five rolling accumulator slots per point, for output planes P-2 .. P+2:
+// every 6x6 window it reads from LDS feeds up to 25 taps x 5 planes x 4
+// points.  Per output the taps accumulate in (dk, dj, di) lexicographic
+// order, the first as a multiply -- the same arithmetic as the per-brick
+// kernel, so results are identical.
+template <class T>
+__global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
+  using A = typename acc_of<T>::type;
+  constexpr int PW = 20, NCELL = PW * PW, NLD = (NCELL + 63) / 64;
+  __shared__ A plane[2][NCELL];
+  __shared__ uint32_t tbl[16];
+
+  const int lane = threadIdx.x;
+  const int lj = lane >> 3, li = lane & 7;
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;
+  const uint32_t *__restrict__ adj = a.adj;
+  const int L = (int)a.len[u];
+  const uint32_t *mk = a.mask + (size_t)u * a.kcol;
+  const size_t st = a.in_step, ost = a.out_step;
+
+  const uint32_t m0 = a.anchor[u];
+  const uint32_t m1 = adj[(size_t)m0 * 27 + 14];
+  const uint32_t m2 = adj[(size_t)m0 * 27 + 16];
+  const uint32_t m3 = adj[(size_t)m1 * 27 + 16];
+
+  // input-level brick table for level -1: the k-1 neighbours of the 4x4
+  // block of columns around the quad (each member's in-plane neighbours)
+  if (lane < 36) {
+    const int mm = lane / 9, q = lane - mm * 9, jd = q / 3, id = q - jd * 3;
+    const uint32_t m = mm == 0 ? m0 : (mm == 1 ? m1 : (mm == 2 ? m2 : m3));
+    const uint32_t n0 = adj[(size_t)m * 27 + 9 + 3 * jd + id];
+    tbl[((mm >> 1) + jd) * 4 + (mm & 1) + id] = adj[(size_t)n0 * 27 + 4];
+  }
+  __syncthreads();
+
+  int eoff[NLD], tix[NLD];
+  bool cv[NLD];
+  size_t base[NLD];
+#pragma unroll
+  for (int t = 0; t < NLD; ++t) {
+    const int q = t * 64 + lane;
+    cv[t] = q < NCELL;
+    const int qq = cv[t] ? q : 0;
+    const int J = qq / PW - 2, I = qq % PW - 2;
+    tix[t] = ((J + 8) >> 3) * 4 + ((I + 8) >> 3);
+    eoff[t] = ((J + 8) & 7) * 8 + ((I + 8) & 7);
+    base[t] = (size_t)tbl[tix[t]] * st;
+  }
+  uint32_t tn = 0;  // next level's table entry (lanes 0..15)
+  if (lane < 16) tn = adj[(size_t)tbl[lane] * 27 + 22];
+
+  const int mm_l = (lj >> 2) * 2 + (li >> 2);
+  const int jo = (2 * lj) & 7, io = (2 * li) & 7;
+  uint32_t om0 = m0, om1 = m1, om2 = m2, om3 = m3;
+
+  A nv[NLD];
+#pragma unroll
+  for (int t = 0; t < NLD; ++t) nv[t] = cv[t] ? widen(a.in[base[t] + 6 * 64 + eoff[t]]) : A(0);
+#pragma unroll
+  for (int t = 0; t < NLD; ++t)
+    if (cv[t]) plane[0][t * 64 + lane] = nv[t];
+  __syncthreads();
+
+  A acc[5][2][2];
+#pragma unroll
+  for (int s = 0; s < 5; ++s)
+#pragma unroll
+    for (int pj = 0; pj < 2; ++pj)
+#pragma unroll
+      for (int pi = 0; pi < 2; ++pi) acc[s][pj][pi] = A(0);
+
+  const int nplanes = 8 * L + 4;
+  const int nout = 8 * L;
+  for (int it = 0; it < nplanes; ++it) {
+    const int P = it - 2;
+    const int buf = it & 1;
+    const bool have_next = it + 1 < nplanes;
+    if (have_next) {
+      const int Pn = P + 1;
+      if ((Pn & 7) == 0) {  // next plane starts a new input level
+        if (lane < 16) tbl[lane] = tn;
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NLD; ++t) base[t] = (size_t)tbl[tix[t]] * st;
+        if (lane < 16) tn = adj[(size_t)tbl[lane] * 27 + 22];
+      }
+      const int kp = Pn & 7;
+#pragma unroll
+      for (int t = 0; t < NLD; ++t)
+        nv[t] = cv[t] ? widen(a.in[base[t] + kp * 64 + eoff[t]]) : A(0);
+    }
+
+    const A *pl = plane[buf];
+    // Coefficients are read group by group (row r, slot s: ten values)
+    // straight from the kernel-argument segment through a constant-address
+    // pointer, so they arrive by scalar loads into SGPRs; the empty asm at
+    // each group hides the pointer's loop invariance so the compiler does
+    // not try to keep all 125 (250 SGPRs) live across the plane loop.  The
+    // argument struct is the kernel's only parameter and sits at offset 0.
+    kconst_ptr<A> cp = (kconst_ptr<A>)((kconst_ptr<char>)__builtin_amdgcn_kernarg_segment_ptr() +
+                                       offsetof(CubeArgs<T>, c));
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      A x[6];
+      // 16-byte LDS reads of the lane's two adjacent columns: 8 lanes of a
+      // row cover 128 contiguous bytes, so a read is free of bank conflicts
+#pragma unroll
+      for (int c = 0; c < 6; c += 2) {
+        typedef A A2 __attribute__((ext_vector_type(2)));
+        const A2 v = *reinterpret_cast<const A2 *>(&pl[(2 * lj + r) * PW + 2 * li + c]);
+        x[c] = v.x;
+        x[c + 1] = v.y;
+      }
+#pragma unroll
+      for (int s = 0; s < 5; ++s) {
+        asm volatile("" : "+s"(cp));
+        const int O = P - 2 + s;
+        if (O < 0 || O >= nout) continue;
+        const int dk = 2 - s;
+#pragma unroll
+        for (int pj = 0; pj < 2; ++pj) {
+          const int dj = r - pj - 2;
+          if (dj < -2 || dj > 2) continue;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+#pragma unroll
+            for (int pi = 0; pi < 2; ++pi) {
+              const int di = c - pi - 2;
+              if (di < -2 || di > 2) continue;
+              const A cf = cp[(dk + 2) * 25 + (dj + 2) * 5 + (di + 2)];
+              if (s == 4 && dj == -2 && di == -2) acc[s][pj][pi] = cf * x[c];
+              else acc[s][pj][pi] = fmadd(cf, x[c], acc[s][pj][pi]);
+            }
+          }
+        }
+      }
+    }
+
+    const int O = P - 2;  // slot 0 is complete
+    if (O >= 0 && O < nout) {
+      const int lo = O >> 3, kk = O & 7;
+      if (kk == 0 && lo > 0) {
+        om0 = adj[(size_t)om0 * 27 + 22];
+        om1 = adj[(size_t)om1 * 27 + 22];
+        om2 = adj[(size_t)om2 * 27 + 22];
+        om3 = adj[(size_t)om3 * 27 + 22];
+      }
+      if ((mk[lo] >> mm_l) & 1u) {
+        const uint32_t ob = mm_l == 0 ? om0 : (mm_l == 1 ? om1 : (mm_l == 2 ? om2 : om3));
+        T *po = a.out + (size_t)ob * ost + kk * 64 + jo * 8 + io;
+        narrow(po[0], acc[0][0][0]);
+        narrow(po[1], acc[0][0][1]);
+        narrow(po[8], acc[0][1][0]);
+        narrow(po[9], acc[0][1][1]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int pj = 0; pj < 2; ++pj)
+#pragma unroll
+        for (int pi = 0; pi < 2; ++pi) acc[s][pj][pi] = acc[s + 1][pj][pi];
+
+    if (have_next) {
+#pragma unroll
+      for (int t = 0; t < NLD; ++t)
+        if (cv[t]) plane[buf ^ 1][t * 64 + lane] = nv[t];
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+hipError_t launch_quads_t(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  using A = typename acc_of<T>::type;
+  CubeArgs<T> a;
+  a.adj = L.adj;
+  a.list = nullptr;
+  a.in = static_cast<const T *>(L.in);
+  a.out = static_cast<T *>(L.out);
+  a.in_step = L.in_step;
+  a.out_step = L.out_step;
+  a.nlist = 0;
+  a.anchor = P.anchor;
+  a.len = P.len;
+  a.mask = P.mask;
+  a.nunits = P.nunits;
+  a.kcol = P.kcol;
+  for (int q = 0; q < 125; ++q) a.c[q] = (A)L.coeff[q];
+  if (P.nunits == 0) return hipSuccess;
+  hipLaunchKernelGGL((cube_sweep_quads<T>), dim3(P.nunits), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 template <class T>
 hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   using A = typename acc_of<T>::type;
@@ -101,6 +316,8 @@ hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   a.in_step = L.in_step;
   a.out_step = L.out_step;
   a.nlist = L.nlist;
+  a.anchor = a.len = a.mask = nullptr;
+  a.nunits = a.kcol = 0;
   for (int q = 0; q < 125; ++q) a.c[q] = (A)L.coeff[q];
   const uint32_t nwg = (L.nlist + 3u) / 4u;
   hipLaunchKernelGGL((cube_sweep_lds<T>), dim3(nwg), dim3(256), 0, s, a);
@@ -114,6 +331,15 @@ hipError_t launch_cube(const SweepLaunch &L, hipStream_t s) {
     case BK_F64: return launch_t<double>(L, s);
     case BK_F32: return launch_t<float>(L, s);
     case BK_BF16: return launch_t<bf16>(L, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_cube_quads(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  switch (L.dtype) {
+    case BK_F64: return launch_quads_t<double>(L, P, s);
+    case BK_F32: return launch_quads_t<float>(L, P, s);
+    case BK_BF16: return launch_quads_t<bf16>(L, P, s);
     default: return hipErrorInvalidValue;
   }
 }

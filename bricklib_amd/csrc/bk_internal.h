@@ -26,12 +26,29 @@ struct SweepLaunch {
   int cond_mode;
 };
 
-// Kernel variants (performance options, results identical).
-enum StarVariant { STAR_LDS = 0 };
-enum CubeVariant { CUBE_LDS = 0 };
+// Device view of an execution plan (see bk_plan.hip).
+struct PlanView {
+  int shape;
+  uint32_t kcol;
+  uint32_t nunits;
+  uint32_t ncovered;
+  const uint32_t *anchor;
+  const uint32_t *len;
+  const uint32_t *mask;
+  const uint32_t *flat;
+  const uint32_t *start;
+};
+const PlanView plan_view(const bk_plan_s *p);
 
+// Per-brick kernels over L.list (any list).
 hipError_t launch_star(const SweepLaunch &L, hipStream_t s);
 hipError_t launch_cube(const SweepLaunch &L, hipStream_t s);
+// Marching kernels over a plan: stars over k-columns, the cube over 2x2
+// column quads streamed plane by plane.
+hipError_t launch_star_columns(const SweepLaunch &L, const PlanView &P, hipStream_t s);
+hipError_t launch_cube_quads(const SweepLaunch &L, const PlanView &P, hipStream_t s);
+
+int plan_fail(int code, const char *msg);
 
 hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,
                                 size_t brick_bytes, const uint32_t *pairs,

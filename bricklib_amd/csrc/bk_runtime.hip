@@ -90,17 +90,26 @@ int check_desc(const bk_sweep_desc *d) {
   if (nc == 0) return fail(BK_ERR_ARG, "bk_sweep: unknown stencil %d", d->stencil);
   if (!valid_dtype(d->dtype)) return fail(BK_ERR_ARG, "bk_sweep: unknown dtype %d", d->dtype);
   if (!d->h_coeff) return fail(BK_ERR_ARG, "bk_sweep: null coefficient pointer");
-  if (d->nlist == 0) return BK_OK;
-  if (!d->d_adj || !d->d_list || !d->d_in || !d->d_out)
+  if (d->stencil == BK_COND7 && d->cond_mode != BK_COND_REFERENCE &&
+      d->cond_mode != BK_COND_THRESHOLD)
+    return fail(BK_ERR_ARG, "bk_sweep: unknown cond mode %d", d->cond_mode);
+  if (!d->plan && d->nlist == 0) return BK_OK;
+  if (!d->d_adj || !d->d_in || !d->d_out || (!d->plan && !d->d_list))
     return fail(BK_ERR_ARG, "bk_sweep: null device pointer");
   if (d->d_in == d->d_out)
     return fail(BK_ERR_ARG, "bk_sweep: input and output fields alias (use ping-pong fields)");
   if (d->in_step < 512 || d->out_step < 512)
     return fail(BK_ERR_ARG, "bk_sweep: step smaller than a brick (512 elements)");
-  if (d->stencil == BK_COND7 && d->cond_mode != BK_COND_REFERENCE &&
-      d->cond_mode != BK_COND_THRESHOLD)
-    return fail(BK_ERR_ARG, "bk_sweep: unknown cond mode %d", d->cond_mode);
   return BK_OK;
+}
+
+bool desc_empty(const bk_sweep_desc *d) {
+  if (d->plan) {
+    uint32_t n = 0;
+    bk_plan_query(d->plan, nullptr, nullptr, nullptr, &n);
+    return n == 0;
+  }
+  return d->nlist == 0;
 }
 
 void to_launch(const bk_sweep_desc *d, bk::SweepLaunch &L) {
@@ -119,7 +128,18 @@ void to_launch(const bk_sweep_desc *d, bk::SweepLaunch &L) {
   L.cond_mode = d->cond_mode;
 }
 
-hipError_t launch_sweep(const bk::SweepLaunch &L, hipStream_t s) {
+hipError_t launch_sweep(const bk::SweepLaunch &L, const bk_plan_s *plan, hipStream_t s) {
+  if (plan) {
+    const bk::PlanView P = bk::plan_view(plan);
+    if (P.ncovered == 0) return hipSuccess;
+    if (L.stencil == BK_CUBE125 && P.shape == BK_PLAN_QUAD) return bk::launch_cube_quads(L, P, s);
+    if (L.stencil != BK_CUBE125 && P.shape == BK_PLAN_COLUMN)
+      return bk::launch_star_columns(L, P, s);
+    bk::SweepLaunch F = L;  // other pairings: per-brick kernel over the plan's bricks
+    F.list = P.flat;
+    F.nlist = P.ncovered;
+    return F.stencil == BK_CUBE125 ? bk::launch_cube(F, s) : bk::launch_star(F, s);
+  }
   if (L.nlist == 0) return hipSuccess;
   if (L.stencil == BK_CUBE125) return bk::launch_cube(L, s);
   return bk::launch_star(L, s);
@@ -145,7 +165,7 @@ hipError_t enqueue_sweeps(const bk_sweep_desc *d, const uint32_t *pairs, uint32_
     L.in_step = sst;
     L.out = dst;
     L.out_step = dstep;
-    hipError_t e = launch_sweep(L, s);
+    hipError_t e = launch_sweep(L, d->plan, s);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -156,6 +176,10 @@ void grid_dims(const long nb[3], long S[3]) {
 }
 
 }  // namespace
+
+namespace bk {
+int plan_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+}  // namespace bk
 
 extern "C" {
 
@@ -406,10 +430,10 @@ int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n) {
 int bk_sweep(const bk_sweep_desc *d, bk_stream s) {
   int rc = check_desc(d);
   if (rc != BK_OK) return rc;
-  if (d->nlist == 0) return BK_OK;
+  if (desc_empty(d)) return BK_OK;
   bk::SweepLaunch L;
   to_launch(d, L);
-  BK_HIP(launch_sweep(L, S(s)));
+  BK_HIP(launch_sweep(L, d->plan, S(s)));
   return BK_OK;
 }
 
@@ -432,7 +456,7 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, 
   if (rc != BK_OK) return rc;
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps: null pair list");
-  if (nsweeps == 0 || d->nlist == 0) return BK_OK;
+  if (nsweeps == 0 || desc_empty(d)) return BK_OK;
   if (!use_graph) {
     BK_HIP(enqueue_sweeps(d, d_pairs, npairs, nsweeps, S(s)));
     return BK_OK;
@@ -452,6 +476,7 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, 
   put(&d->in_step, sizeof(d->in_step));
   put(&d->d_out, sizeof(d->d_out));
   put(&d->out_step, sizeof(d->out_step));
+  put(&d->plan, sizeof(d->plan));
   put(d->h_coeff, sizeof(double) * (size_t)bk_stencil_ncoeff(d->stencil));
   put(&d->theta, sizeof(d->theta));
   put(&d->cond_mode, sizeof(d->cond_mode));

@@ -37,6 +37,14 @@ struct StarArgs {
   uint64_t out_step;
   uint32_t nlist;
   int cond_mode;
+  // plan (column kernel only)
+  const uint32_t *__restrict__ anchor;
+  const uint32_t *__restrict__ len;
+  const uint32_t *__restrict__ mask;
+  const uint32_t *__restrict__ flat;
+  const uint32_t *__restrict__ start;
+  uint32_t nunits;
+  uint32_t kcol;
   A theta;
   A c[6 * R + 1];
 };
@@ -125,6 +133,246 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
   }
 }
 
+
+// Column-marching variant: one wave (one workgroup) per plan unit, a run of
+// up to kcol bricks b, adj[b][22], ... along k.  While brick l is computed,
+// brick l+1 is already being loaded into registers; its plane 0 is the k+1
+// face of brick l, and the top R planes of brick l become the k-1 face of
+// brick l+1, so no k-face is read twice.  i/j faces come through the LDS
+// tile as in the per-brick kernel.  Same per-point arithmetic.
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, COND> a) {
+  using A = typename acc_of<T>::type;
+  constexpr int P = 8 + 2 * R;
+  __shared__ A tile[8][P][P];
+
+  const int lane = threadIdx.x;
+  const int j = lane >> 3, i = lane & 7;
+  const int fk = lane >> 3, fl = lane & 7;
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;
+  const uint32_t L = a.len[u];
+  const uint32_t *mk = a.mask + (size_t)u * a.kcol;
+  const size_t st = a.in_step;
+
+  uint32_t b = a.anchor[u];
+  A cur[8], nxt[8], km[R];
+  {
+    const T *p = a.in + (size_t)b * st;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cur[k] = widen(p[k * 64 + lane]);
+    const T *q = a.in + (size_t)a.adj[(size_t)b * 27 + 4] * st;
+#pragma unroll
+    for (int r = 0; r < R; ++r) km[r] = widen(q[(8 - R + r) * 64 + lane]);
+  }
+  for (uint32_t l = 0; l < L; ++l) {
+    const uint32_t *ab = a.adj + (size_t)b * 27;
+    const uint32_t up = ab[22];
+    const bool more = l + 1 < L;
+    // Face loads first: loads retire in order, so the faces can be waited
+    // for while the prefetch of the next brick (issued after) stays in
+    // flight through the compute below.
+    const T *pjm = a.in + (size_t)ab[10] * st;
+    const T *pjp = a.in + (size_t)ab[16] * st;
+    const T *pim = a.in + (size_t)ab[12] * st;
+    const T *pip = a.in + (size_t)ab[14] * st;
+    A fjm[R], fjp[R], fim[R], fip[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      fjm[r] = widen(pjm[fk * 64 + (8 - R + r) * 8 + fl]);
+      fjp[r] = widen(pjp[fk * 64 + r * 8 + fl]);
+      fim[r] = widen(pim[fk * 64 + fl * 8 + (8 - R + r)]);
+      fip[r] = widen(pip[fk * 64 + fl * 8 + r]);
+    }
+    const T *pu = a.in + (size_t)up * st;
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) nxt[k] = widen(pu[k * 64 + lane]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) nxt[r] = widen(pu[r * 64 + lane]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) tile[k][R + j][R + i] = cur[k];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      tile[fk][r][R + fl] = fjm[r];
+      tile[fk][8 + R + r][R + fl] = fjp[r];
+      tile[fk][R + fl][r] = fim[r];
+      tile[fk][R + fl][8 + R + r] = fip[r];
+    }
+    __syncthreads();
+    if (mk[l] & 1u) {
+      T *po = a.out + (size_t)b * a.out_step;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        auto X = [&](int kk) -> A {
+          return kk < 0 ? km[kk + R] : (kk >= 8 ? nxt[kk - 8] : cur[kk]);
+        };
+        A acc = a.c[0] * tapv<A, COND>(cur[k]);
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+          acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[k][R + j][R + i + r]), acc);
+          acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[k][R + j][R + i - r]), acc);
+          acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[k][R + j + r][R + i]), acc);
+          acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[k][R + j - r][R + i]), acc);
+          acc = fmadd(a.c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
+          acc = fmadd(a.c[6 * r], tapv<A, COND>(X(k - r)), acc);
+        }
+        if constexpr (COND) {
+          const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (cur[k] > a.theta);
+          acc = sel ? acc : -acc;
+        }
+        narrow(po[k * 64 + lane], acc);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) km[r] = cur[8 - R + r];
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
+    }
+    b = up;
+    __syncthreads();
+  }
+}
+
+
+// K-stack variant: one workgroup of four waves per column unit of up to
+// four bricks stacked along k (plan kcol <= 4).  The four waves load their
+// bricks at the same time (the memory parallelism of the per-brick kernel)
+// into one tall LDS tile [32][8+2R][8+2R]; the k-faces between the stacked
+// bricks are read from that tile instead of from memory, so only the unit's
+// bottom and top faces are fetched.  Same per-point arithmetic.
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(256) void star_sweep_kstack(const StarArgs<T, R, COND> a) {
+  using A = typename acc_of<T>::type;
+  constexpr int P = 8 + 2 * R, W = 4;
+  __shared__ A tile[W * 8][P][P];
+  __shared__ A kb[R][64], ka[R][64];
+
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int j = lane >> 3, i = lane & 7;
+  const int fk = lane >> 3, fl = lane & 7;
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;  // uniform over the workgroup
+  const int L = (int)a.len[u];
+  const bool active = w < L;
+  const size_t st = a.in_step;
+
+  A v[8];
+  uint32_t b = 0;
+  if (active) {
+    b = uniform(a.flat[a.start[u] + (uint32_t)w]);
+    const uint32_t *ab = a.adj + (size_t)b * 27;
+    const T *pc = a.in + (size_t)b * st;
+    const T *pjm = a.in + (size_t)uniform(ab[10]) * st;
+    const T *pjp = a.in + (size_t)uniform(ab[16]) * st;
+    const T *pim = a.in + (size_t)uniform(ab[12]) * st;
+    const T *pip = a.in + (size_t)uniform(ab[14]) * st;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = widen(pc[k * 64 + lane]);
+    A fjm[R], fjp[R], fim[R], fip[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      fjm[r] = widen(pjm[fk * 64 + (8 - R + r) * 8 + fl]);
+      fjp[r] = widen(pjp[fk * 64 + r * 8 + fl]);
+      fim[r] = widen(pim[fk * 64 + fl * 8 + (8 - R + r)]);
+      fip[r] = widen(pip[fk * 64 + fl * 8 + r]);
+    }
+    if (w == 0) {
+      const T *q = a.in + (size_t)uniform(ab[4]) * st;
+#pragma unroll
+      for (int r = 0; r < R; ++r) kb[r][lane] = widen(q[(8 - R + r) * 64 + lane]);
+    }
+    if (w == L - 1) {
+      const T *q = a.in + (size_t)uniform(ab[22]) * st;
+#pragma unroll
+      for (int r = 0; r < R; ++r) ka[r][lane] = widen(q[r * 64 + lane]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) tile[w * 8 + k][R + j][R + i] = v[k];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      tile[w * 8 + fk][r][R + fl] = fjm[r];
+      tile[w * 8 + fk][8 + R + r][R + fl] = fjp[r];
+      tile[w * 8 + fk][R + fl][r] = fim[r];
+      tile[w * 8 + fk][R + fl][8 + R + r] = fip[r];
+    }
+  }
+  __syncthreads();
+  if (!active || !(a.mask[(size_t)u * a.kcol + w] & 1u)) return;
+
+  const int top = L * 8;
+  T *po = a.out + (size_t)b * a.out_step;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    auto X = [&](int kk) -> A {
+      if (kk >= 0 && kk < 8) return v[kk];
+      const int gk = w * 8 + kk;  // plane in the stack
+      if (gk < 0) return kb[gk + R][lane];
+      if (gk >= top) return ka[gk - top][lane];
+      return tile[gk][R + j][R + i];
+    };
+    A acc = a.c[0] * tapv<A, COND>(v[k]);
+#pragma unroll
+    for (int r = 1; r <= R; ++r) {
+      acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[w * 8 + k][R + j][R + i + r]), acc);
+      acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[w * 8 + k][R + j][R + i - r]), acc);
+      acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[w * 8 + k][R + j + r][R + i]), acc);
+      acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[w * 8 + k][R + j - r][R + i]), acc);
+      acc = fmadd(a.c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
+      acc = fmadd(a.c[6 * r], tapv<A, COND>(X(k - r)), acc);
+    }
+    if constexpr (COND) {
+      const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (v[k] > a.theta);
+      acc = sel ? acc : -acc;
+    }
+    narrow(po[k * 64 + lane], acc);
+  }
+}
+
+template <class T, int R, bool COND>
+hipError_t launch_cols_t(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  using A = typename acc_of<T>::type;
+  StarArgs<T, R, COND> a;
+  a.adj = L.adj;
+  a.list = nullptr;
+  a.in = static_cast<const T *>(L.in);
+  a.out = static_cast<T *>(L.out);
+  a.in_step = L.in_step;
+  a.out_step = L.out_step;
+  a.nlist = 0;
+  a.cond_mode = L.cond_mode;
+  a.anchor = P.anchor;
+  a.len = P.len;
+  a.mask = P.mask;
+  a.flat = P.flat;
+  a.start = P.start;
+  a.nunits = P.nunits;
+  a.kcol = P.kcol;
+  a.theta = (A)L.theta;
+  for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
+  if (P.nunits == 0) return hipSuccess;
+  if (P.kcol <= 4)
+    hipLaunchKernelGGL((star_sweep_kstack<T, R, COND>), dim3(P.nunits), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((star_sweep_columns<T, R, COND>), dim3(P.nunits), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_cols_dt(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  switch (L.stencil) {
+    case BK_STAR7: return launch_cols_t<T, 1, false>(L, P, s);
+    case BK_STAR13: return launch_cols_t<T, 2, false>(L, P, s);
+    case BK_STAR25: return launch_cols_t<T, 4, false>(L, P, s);
+    case BK_COND7: return launch_cols_t<T, 1, true>(L, P, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <class T, int R, bool COND>
 hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   using A = typename acc_of<T>::type;
@@ -137,6 +385,8 @@ hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   a.out_step = L.out_step;
   a.nlist = L.nlist;
   a.cond_mode = L.cond_mode;
+  a.anchor = a.len = a.mask = a.flat = a.start = nullptr;
+  a.nunits = a.kcol = 0;
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
   const uint32_t nwg = (L.nlist + 3u) / 4u;
@@ -162,6 +412,15 @@ hipError_t launch_star(const SweepLaunch &L, hipStream_t s) {
     case BK_F64: return launch_dt<double>(L, s);
     case BK_F32: return launch_dt<float>(L, s);
     case BK_BF16: return launch_dt<bf16>(L, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_star_columns(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  switch (L.dtype) {
+    case BK_F64: return launch_cols_dt<double>(L, P, s);
+    case BK_F32: return launch_cols_dt<float>(L, P, s);
+    case BK_BF16: return launch_cols_dt<bf16>(L, P, s);
     default: return hipErrorInvalidValue;
   }
 }

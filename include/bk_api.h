@@ -73,6 +73,9 @@ enum bk_cond_mode { BK_COND_REFERENCE = 0, BK_COND_THRESHOLD = 1 };
 
 typedef void *bk_stream; /* hipStream_t */
 typedef void *bk_event;  /* hipEvent_t */
+typedef struct bk_plan_s *bk_plan; /* execution plan (bk_plan_create) */
+
+enum bk_plan_shape { BK_PLAN_COLUMN = 1, BK_PLAN_QUAD = 4 };
 
 /* ---- library, errors, device ------------------------------------------ */
 int bk_version(void);
@@ -135,6 +138,43 @@ int bk_ghost_pairs(const long nb[3], uint32_t *h_pairs, uint32_t *npairs);
  * h_list may be NULL to query *n. */
 int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n);
 
+/* ---- execution plans --------------------------------------------------- */
+/* A plan is a schedule for sweeping the bricks of a list: it groups them,
+ * through the host copy of the adjacency, into units one wave marches
+ * through along k -- k-columns of up to kcol bricks (BK_PLAN_COLUMN) or
+ * up to kcol levels of 2x2 blocks of columns (BK_PLAN_QUAD; members
+ * anchor, adj[anchor][14], adj[anchor][16], adj[adj[anchor][14]][16]).
+ * Every listed brick is written exactly once by a sweep that uses the plan;
+ * bricks a unit computes but that are not listed are not written.  The
+ * reference has no counterpart: its sweep is a loop over grid[k][j][i]
+ * (stencils/3axis.cpp:51-66, weak/main.hip.cpp:274-279) and the plan is
+ * how that loop is scheduled on the GPU.  Allocates device memory.
+ * Kernel choice follows the plan: star stencils run a column plan with
+ * kcol <= 4 as k-stacks (one workgroup loads the stacked bricks together
+ * and shares their k-faces in LDS) and with kcol > 4 as k-marches (one wave
+ * walks the column, prefetching the next brick); the cube streams a quad
+ * plan plane by plane; any other pairing, and a sweep without a plan, runs
+ * one brick per wave.  Results do not depend on the choice. */
+int bk_plan_create(const uint32_t *h_adj, uint32_t nbricks,
+                   const uint32_t *h_list, uint32_t nlist, int shape,
+                   uint32_t kcol, bk_plan *plan);
+int bk_plan_destroy(bk_plan plan);
+/* The same schedule built on the host only (no device memory): h_anchor and
+ * h_len hold nunits <= nlist entries, h_mask nunits*kcol, h_flat ncovered
+ * (== the number of distinct listed bricks).  Any array may be NULL to
+ * query the counts first. */
+int bk_plan_build_host(const uint32_t *h_adj, uint32_t nbricks,
+                       const uint32_t *h_list, uint32_t nlist, int shape,
+                       uint32_t kcol, uint32_t *h_anchor, uint32_t *h_len,
+                       uint32_t *h_mask, uint32_t *h_flat, uint32_t *nunits,
+                       uint32_t *ncovered);
+int bk_plan_query(bk_plan plan, uint32_t *nunits, uint32_t *kcol, int *shape,
+                  uint32_t *ncovered);
+/* Copy the plan's arrays to the host (any pointer may be NULL):
+ * anchor[nunits], len[nunits], mask[nunits*kcol], flat[ncovered]. */
+int bk_plan_download(bk_plan plan, uint32_t *h_anchor, uint32_t *h_len,
+                     uint32_t *h_mask, uint32_t *h_flat);
+
 /* ---- the sweep -------------------------------------------------------- */
 typedef struct bk_sweep_desc {
   int stencil;            /* enum bk_stencil */
@@ -149,9 +189,12 @@ typedef struct bk_sweep_desc {
   const double *h_coeff;  /* host coefficients, bk_stencil_ncoeff values */
   double theta;           /* COND7 threshold (threshold form) */
   int cond_mode;          /* enum bk_cond_mode */
+  bk_plan plan;           /* optional: when set, the sweep covers the plan's
+                             bricks and d_list / nlist are ignored */
 } bk_sweep_desc;
 
-/* One stencil sweep: for every brick b in d_list and every element of b,
+/* One stencil sweep: for every brick b in d_list (or in the plan) and
+ * every element of b,
  * out[b] = stencil(in) read through the adjacency.  This is the loop of
  * stencils/3axis.cpp:51-66 (7pt), :123-140 (cond), and the brick(...)
  * kernel of weak/main.hip.cpp:32-40 (13/25/125-point via

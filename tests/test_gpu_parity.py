@@ -81,6 +81,48 @@ def test_fp64_within_1e12_of_reference_order(gpu, stencil):
 
 
 @pytest.mark.parametrize("stencil", STENCILS, ids=lambda s: NAMES[s])
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32", "bf16"])
+@pytest.mark.parametrize("plan", [None, ("column", 4), ("column", 3), ("column", 8),
+                                  ("quad", 2), ("quad", 8)],
+                         ids=["list", "kstack4", "kstack3", "kmarch8", "quad2", "quad8"])
+def test_one_sweep_bit_exact_plans(gpu, stencil, dtype, plan):
+    """Every schedule gives the oracle's bits: per-brick list, k-stacks,
+    k-column marching, 2x2 column quads (odd dims give partial quads)."""
+    print(f"[case] plan={plan} {NAMES[stencil]} dtype={dtype} nb=(3,4,5)", flush=True)
+    st = S.Stencil.make(stencil, seed=11)
+    g, a, b, og, oa = setup(gpu, (3, 4, 5), dtype, 13)
+    g.ghost_refresh(a)
+    pl = None if plan is None else g.plan(plan[0], kcol=plan[1])
+    gpu.sweep(st, b, a, g, plan=pl)
+    og.refresh(oa, dtype)
+    ob = og.storage(dtype)
+    og.sweep(stencil, st.coeff, oa, ob, dtype=dtype, theta=st.theta, cond_mode=st.cond_mode)
+    np.testing.assert_array_equal(host(b)[og.interior], ob[og.interior])
+    # bricks outside the interior list are never written
+    ghosts = np.setdiff1d(np.arange(og.nbricks), og.interior)
+    assert not host(b)[ghosts].any()
+
+
+@pytest.mark.parametrize("shape", ["column", "quad"])
+def test_plan_of_shuffled_list(gpu, shape):
+    """A plan built from a shuffled interior list (broken chains, scattered
+    quads) still writes exactly the oracle's values."""
+    nb = (4, 4, 4)
+    stencil = S.BK_CUBE125 if shape == "quad" else S.BK_STAR25
+    st = S.Stencil.make(stencil, seed=2)
+    g, a, b, og, oa = setup(gpu, nb, 0, 5)
+    g.ghost_refresh(a)
+    blist = g.interior.copy()
+    np.random.default_rng(1).shuffle(blist)
+    pl = gpu.Plan(g.info, blist, shape, 4)
+    gpu.sweep(st, b, a, g, plan=pl)
+    og.refresh(oa)
+    ob = og.storage()
+    og.sweep(stencil, st.coeff, oa, ob)
+    np.testing.assert_array_equal(host(b)[og.interior], ob[og.interior])
+
+
+@pytest.mark.parametrize("stencil", STENCILS, ids=lambda s: NAMES[s])
 @pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
 def test_pingpong_sweeps_bit_exact(gpu, stencil, use_graph):
     print(f"[case] 8 sweeps {NAMES[stencil]} graph={use_graph}", flush=True)
