@@ -25,6 +25,27 @@ __global__ __launch_bounds__(256) void ghost_copy16(char *__restrict__ base,
   for (uint32_t v = lane; v < vec_per_brick; v += 64) d[v] = s[v];
 }
 
+// Whole-brick copy with NV 16-byte vectors per lane, every load issued
+// before the first store (the generic loop above alternates them, since a
+// store may alias the next load).
+template <int NV>
+__global__ __launch_bounds__(256) void ghost_copy16_n(char *__restrict__ base,
+                                                      uint64_t step_bytes,
+                                                      const uint32_t *__restrict__ pairs,
+                                                      uint32_t npairs) {
+  const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (p >= npairs) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t dst = pairs[2 * p], src = pairs[2 * p + 1];
+  const uint4 *s = reinterpret_cast<const uint4 *>(base + (size_t)src * step_bytes);
+  uint4 *d = reinterpret_cast<uint4 *>(base + (size_t)dst * step_bytes);
+  uint4 t[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) t[q] = s[lane + 64 * q];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) d[lane + 64 * q] = t[q];
+}
+
 __global__ __launch_bounds__(256) void ghost_copy2(char *__restrict__ base,
                                                    uint64_t step_bytes,
                                                    uint32_t half_per_brick,
@@ -37,6 +58,26 @@ __global__ __launch_bounds__(256) void ghost_copy2(char *__restrict__ base,
   const uint16_t *s = reinterpret_cast<const uint16_t *>(base + (size_t)src * step_bytes);
   uint16_t *d = reinterpret_cast<uint16_t *>(base + (size_t)dst * step_bytes);
   for (uint32_t v = lane; v < half_per_brick; v += 64) d[v] = s[v];
+}
+
+// ---------------------------------------------------------------------------
+// streaming copy used to measure achievable HBM bandwidth: 16-byte vectors,
+// four loads in flight per lane before their (non-temporal) stores
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void stream_copy16(const u32x4 *__restrict__ s,
+                                                     u32x4 *__restrict__ d, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 1024;
+  size_t q = (size_t)blockIdx.x * 1024 + threadIdx.x;
+  for (; q + 768 < n; q += stride) {
+    const u32x4 t0 = s[q], t1 = s[q + 256], t2 = s[q + 512], t3 = s[q + 768];
+    __builtin_nontemporal_store(t0, d + q);
+    __builtin_nontemporal_store(t1, d + q + 256);
+    __builtin_nontemporal_store(t2, d + q + 512);
+    __builtin_nontemporal_store(t3, d + q + 768);
+  }
+  for (; q < n; q += 256) d[q] = s[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -241,12 +282,36 @@ hipError_t launch_ghost_refresh(void *dat, size_t step_bytes, size_t brick_bytes
   const uint32_t nwg = (npairs + 3u) / 4u;
   const bool v16 = (reinterpret_cast<uintptr_t>(dat) % 16 == 0) && (step_bytes % 16 == 0) &&
                    (brick_bytes % 16 == 0);
-  if (v16)
-    hipLaunchKernelGGL(ghost_copy16, dim3(nwg), dim3(256), 0, s, static_cast<char *>(dat),
-                       (uint64_t)step_bytes, (uint32_t)(brick_bytes / 16), pairs, npairs);
+  char *b = static_cast<char *>(dat);
+  if (v16 && brick_bytes == 4096)
+    hipLaunchKernelGGL(ghost_copy16_n<4>, dim3(nwg), dim3(256), 0, s, b, (uint64_t)step_bytes,
+                       pairs, npairs);
+  else if (v16 && brick_bytes == 2048)
+    hipLaunchKernelGGL(ghost_copy16_n<2>, dim3(nwg), dim3(256), 0, s, b, (uint64_t)step_bytes,
+                       pairs, npairs);
+  else if (v16 && brick_bytes == 1024)
+    hipLaunchKernelGGL(ghost_copy16_n<1>, dim3(nwg), dim3(256), 0, s, b, (uint64_t)step_bytes,
+                       pairs, npairs);
+  else if (v16)
+    hipLaunchKernelGGL(ghost_copy16, dim3(nwg), dim3(256), 0, s, b, (uint64_t)step_bytes,
+                       (uint32_t)(brick_bytes / 16), pairs, npairs);
   else
     hipLaunchKernelGGL(ghost_copy2, dim3(nwg), dim3(256), 0, s, static_cast<char *>(dat),
                        (uint64_t)step_bytes, (uint32_t)(brick_bytes / 2), pairs, npairs);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  if (n == 0) return hipSuccess;
+  int dev = 0, cus = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  size_t nblk = (n + 1023) / 1024;
+  const size_t cap = (size_t)cus * 8;
+  if (nblk > cap) nblk = cap;
+  hipLaunchKernelGGL(stream_copy16, dim3((uint32_t)nblk), dim3(256), 0, s,
+                     static_cast<const u32x4 *>(src), static_cast<u32x4 *>(dst), n);
   return hipGetLastError();
 }
 

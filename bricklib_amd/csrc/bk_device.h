@@ -35,6 +35,16 @@ __device__ __forceinline__ void narrow(bf16 &d, float v) {
   }
 }
 
+// Output stores bypass L2 allocation: a sweep never reads what it writes,
+// and keeping outputs out of L2 leaves it to the neighbour faces.
+__device__ __forceinline__ void store_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(bf16 *p, float v) {
+  bf16 h;
+  narrow(h, v);
+  __builtin_nontemporal_store(h.u, &p->u);
+}
+
 __device__ __forceinline__ double fmadd(double a, double b, double c) {
   return __builtin_fma(a, b, c);
 }

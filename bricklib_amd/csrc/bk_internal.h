@@ -54,6 +54,7 @@ hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,
                                 size_t brick_bytes, const uint32_t *pairs,
                                 uint32_t npairs, hipStream_t s);
 hipError_t launch_zero(void *dst, size_t bytes, hipStream_t s);
+hipError_t launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_fill_uniform(int dtype, void *dat, size_t step,
                                const long nb[3], uint64_t seed, double lo,
                                double hi, hipStream_t s);

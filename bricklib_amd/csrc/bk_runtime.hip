@@ -286,6 +286,18 @@ int bk_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, bk_stream s) {
   return BK_OK;
 }
 
+int bk_stream_copy(void *d_dst, const void *d_src, size_t bytes, bk_stream s) {
+  if (bytes == 0) return BK_OK;
+  if (!d_dst || !d_src) return fail(BK_ERR_ARG, "bk_stream_copy: null pointer");
+  if (bytes % 16 || reinterpret_cast<uintptr_t>(d_dst) % 16 || reinterpret_cast<uintptr_t>(d_src) % 16)
+    return fail(BK_ERR_ARG, "bk_stream_copy: pointers and size must be 16-byte aligned");
+  const char *a = static_cast<const char *>(d_src);
+  const char *b = static_cast<const char *>(d_dst);
+  if (a < b + bytes && b < a + bytes) return fail(BK_ERR_ARG, "bk_stream_copy: ranges overlap");
+  BK_HIP(bk::launch_stream_copy(d_dst, d_src, bytes, S(s)));
+  return BK_OK;
+}
+
 int bk_zero(void *d_dst, size_t bytes, bk_stream s) {
   if (bytes == 0) return BK_OK;
   if (!d_dst) return fail(BK_ERR_ARG, "bk_zero: null pointer");

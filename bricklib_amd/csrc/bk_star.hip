@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
       const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (v[k] > a.theta);
       acc = sel ? acc : -acc;
     }
-    narrow(po[k * 64 + lane], acc);
+    store_nt(&po[k * 64 + lane], acc);
   }
 }
 
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
           const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (cur[k] > a.theta);
           acc = sel ? acc : -acc;
         }
-        narrow(po[k * 64 + lane], acc);
+        store_nt(&po[k * 64 + lane], acc);
       }
     }
 #pragma unroll
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void star_sweep_kstack(const StarArgs<T, R, CO
       const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (v[k] > a.theta);
       acc = sel ? acc : -acc;
     }
-    narrow(po[k * 64 + lane], acc);
+    store_nt(&po[k * 64 + lane], acc);
   }
 }
 

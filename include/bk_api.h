@@ -102,6 +102,10 @@ int bk_alloc(size_t bytes, void **d_ptr);
 int bk_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes, bk_stream s);
 int bk_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes, bk_stream s);
 int bk_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, bk_stream s);
+/* Streaming device copy (16-byte aligned, non-overlapping).  A measurement
+ * utility with no counterpart in the reference: the benchmark uses it as
+ * the achievable-HBM-bandwidth yardstick for the memory-bound sweeps. */
+int bk_stream_copy(void *d_dst, const void *d_src, size_t bytes, bk_stream s);
 /* Zero-fill with a kernel (no hipMemset node, so it is safe in graphs). */
 int bk_zero(void *d_dst, size_t bytes, bk_stream s);
 

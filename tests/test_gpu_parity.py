@@ -266,3 +266,17 @@ def test_aliasing_rejected(gpu):
     g, a, b, og, oa = setup(gpu, (1, 1, 1), 0, 1)
     with pytest.raises(gpu.BrickError):
         gpu.sweep(st, a, a, g)
+
+
+def test_stream_copy(gpu):
+    import ctypes as C
+    from bricklib_amd import _lib
+    lib = gpu.load()
+    for nbytes in (16, 4096 * 3 + 48, 1 << 20):
+        src = np.random.default_rng(nbytes).integers(0, 255, nbytes, dtype=np.uint8)
+        ds = gpu.DeviceBuffer.from_array(src)
+        dd = gpu.DeviceBuffer(nbytes)
+        dd.zero()
+        _lib.check(lib.bk_stream_copy(C.c_void_p(dd.ptr), C.c_void_p(ds.ptr), nbytes, None))
+        np.testing.assert_array_equal(dd.download(np.uint8), src)
+    assert lib.bk_stream_copy(C.c_void_p(dd.ptr), C.c_void_p(dd.ptr), 16, None) == 1
