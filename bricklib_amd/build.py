@@ -71,6 +71,26 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_examples() -> list:
+    """Compile the C++ drop-in example(s) against include/ and libbricks.so
+    with the host compiler (no GPU code), into examples/bin/."""
+    out_dir = os.path.join(ROOT, "examples", "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    built = []
+    for src in sorted(os.listdir(os.path.join(ROOT, "examples"))):
+        if not src.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, src[:-4])
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fopenmp", "-Wall",
+               "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "examples", src),
+               "-L", HERE, "-lbricks", "-Wl,-rpath,$ORIGIN/../../bricklib_amd", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed for {src}:\n{r.stdout}\n{r.stderr}")
+        built.append(exe)
+    return built
+
+
 def build_oracle(with_ref: bool | None = None) -> None:
     """Build the oracle's C library (and oracle/_ref when the reference tree
     exists).  The oracle is test infrastructure; the product never loads it."""
@@ -85,5 +105,6 @@ def build_oracle(with_ref: bool | None = None) -> None:
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_examples()
     build_oracle()
     sys.exit(0)

@@ -1,0 +1,22 @@
+"""The C++ drop-in example (examples/drop_in_7pt.cpp) on the GPU: bricklib
+user code -- init_grid, copyToBrick, movBrickInfo/movBrickStorage, the
+sweep, compareBrick -- against its own host array loop, 1 sweep and 8
+ping-pong sweeps with the on-device periodic ghost refresh."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "examples", "bin", "drop_in_7pt")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,sweeps", [(32, 1), (64, 8), (48, 5)])
+def test_drop_in_example(gpu, n, sweeps):
+    assert os.path.exists(EXE), "examples/bin/drop_in_7pt not built (run build())"
+    r = subprocess.run([EXE, str(n), str(sweeps)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "result match" in r.stdout
