@@ -16,7 +16,10 @@ The same JSON line also carries:
   cpu_baseline -- the oracle's restatement of the reference brick loop
                   (stencils/3axis.cpp:51-66) with OpenMP on the host cores;
   by_stencil   -- 13/25-point, 125-point cube and cond at the same size,
-                  plus the FP32 / BF16 forms of the 7-point star.
+                  plus the FP32 / BF16 forms of the 7-point star;
+  precision    -- FP32 and BF16-storage forms against the FP64 sweep after
+                  1, 10 and 100 sweeps at 512^3 and 1024^3 (errors computed
+                  in FP64 on the GPU) and their speed-up over FP64.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -48,6 +51,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--only", default="", help="comma list of extra cases to run (default all)")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--no-precision", action="store_true")
+    p.add_argument("--precision-1024", action="store_true", default=True,
+                   help="also run the precision study at twice the size (1024^3)")
     return p.parse_args()
 
 
@@ -193,6 +199,7 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
     by = {}
+    cur = {}
     if not args.no_extra:
         only = set(x for x in args.only.split(",") if x)
         cases = [("13pt_f64", "13pt", bk.BK_F64), ("25pt_f64", "25pt", bk.BK_F64),
@@ -221,6 +228,30 @@ def main():
                         "hbm_gbs": round(gb, 1), "hbm_frac": round(gb / HBM_PEAK_GBS, 4),
                         "tflops": round(tf, 2), "valu_frac": round(tf / peak_tf, 4)}
             print(f"[bench] {name}: {by[name]}", file=sys.stderr, flush=True)
+
+    prec = {}
+    if not args.no_precision:
+        from bricklib_amd import precision as PR
+        cur.clear()
+        del a, b
+        for pn in sorted(set([n, 2 * n])) if args.precision_1024 else [n]:
+            for sname, dts in (("7pt", (bk.BK_F32, bk.BK_BF16)), ("125pt", (bk.BK_F32,))):
+                key = f"{sname}_{pn}"
+                prec[key] = PR.study(pn, sname, dtypes=dts, sweeps=(1, 10, 100), stream=stream)
+                gp = bk.PeriodicGrid((pn // 8,) * 3, stream=stream)
+                stp = S.Stencil.make(sname, seed=7)
+                times = {}
+                for dt in (bk.BK_F64, bk.BK_F32) + ((bk.BK_BF16,) if sname == "7pt" else ()):
+                    fa = bk.Brick(gp.info, gp.field_storage(dt), 0)
+                    fb = bk.Brick(gp.info, gp.field_storage(dt), 0)
+                    gp.fill_uniform(fa, 7, stream=stream)
+                    times[dt] = time_sweeps(bk, stp, fa, fb, gp, stream, 20, 2)
+                    del fa, fb
+                prec[key]["ms_per_step"] = {PR.DTYPE_NAMES[d]: round(t, 4) for d, t in times.items()}
+                prec[key]["speedup_vs_f64"] = {PR.DTYPE_NAMES[d]: round(times[bk.BK_F64] / t, 3)
+                                               for d, t in times.items()}
+                del gp
+                print(f"[bench] precision {key}: {prec[key]}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -253,6 +284,7 @@ def main():
                      "frac_of_measured_copy": round(achieved / peak_copy, 4)},
         "cpu_baseline": cpu,
         "by_stencil": by,
+        "precision": prec,
     }
     print(json.dumps(line), flush=True)
 

@@ -12,6 +12,7 @@ from .bricks import (Brick, BrickInfo, BrickStorage, DeviceBuffer, Event, Period
                      Plan, Stream, plan_host, copyFromBrick, copyToBrick, device_count, device_sync, init_grid,
                      movBrickInfo, movBrickStorage, sweep, sweeps)
 from .stencils import Stencil
+from . import precision  # noqa: E402
 
 __all__ = [
     "BK_F64", "BK_F32", "BK_BF16", "BK_STAR7", "BK_STAR13", "BK_STAR25", "BK_CUBE125",

@@ -46,6 +46,27 @@ class bk_sweep_desc(C.Structure):
     ]
 
 
+class bk_batch_req(C.Structure):
+    _fields_ = [
+        ("stencil", C.c_int),
+        ("dtype", C.c_int),
+        ("d_adj", vp),
+        ("d_list", vp),
+        ("nlist", C.c_uint32),
+        ("plan", vp),
+        ("d_pairs", vp),
+        ("npairs", C.c_uint32),
+        ("d_f0", vp),
+        ("d_f1", vp),
+        ("step", C.c_size_t),
+        ("h_coeff", dblp),
+        ("theta", C.c_double),
+        ("cond_mode", C.c_int),
+        ("parity", C.c_uint32),
+        ("ticks", C.c_uint32),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "bk_version": (C.c_int, []),
@@ -59,6 +80,8 @@ SIGNATURES = {
     "bk_storage_alloc": (C.c_int, [C.c_int, C.c_long, C.c_size_t, C.POINTER(vp)]),
     "bk_free": (C.c_int, [vp]),
     "bk_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),
+    "bk_alloc_async": (C.c_int, [C.c_size_t, C.POINTER(vp), vp]),
+    "bk_free_async": (C.c_int, [vp, vp]),
     "bk_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "bk_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "bk_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t, vp]),
@@ -92,6 +115,10 @@ SIGNATURES = {
     "bk_plan_download": (C.c_int, [vp, u32p, u32p, u32p, u32p]),
     "bk_plan_build_host": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, C.c_int, C.c_uint32,
                                      u32p, u32p, u32p, u32p, u32p, u32p]),
+    "bk_batch_create": (C.c_int, [C.c_uint32, C.POINTER(vp)]),
+    "bk_batch_destroy": (C.c_int, [vp]),
+    "bk_batch_set": (C.c_int, [vp, C.POINTER(bk_batch_req), C.c_uint32, vp]),
+    "bk_batch_run": (C.c_int, [vp, C.c_uint32, C.c_int, vp]),
     "bk_copy_to_brick": (C.c_int, [C.c_int, longp, longp, longp, vp, vp, vp, C.c_size_t, vp]),
     "bk_copy_from_brick": (C.c_int, [C.c_int, longp, longp, longp, vp, vp, vp, C.c_size_t, vp]),
 }

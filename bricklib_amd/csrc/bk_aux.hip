@@ -80,6 +80,47 @@ __global__ __launch_bounds__(256) void stream_copy16(const u32x4 *__restrict__ s
   for (; q < n; q += 256) d[q] = s[q];
 }
 
+// Batched periodic refresh: one persistent launch refreshes the input field
+// of every request of a table (all of one dtype); work item = one (dst,
+// src) ghost pair of one request; NV 16-byte vectors per lane per brick.
+template <int NV>
+__global__ __launch_bounds__(256) void ghost_batched(const BatchTable *tab_, uint32_t tick) {
+  const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
+  const uint32_t nsegs = tab->nsegs;
+  const uint32_t total = tab->prefix[nsegs];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t p = blockIdx.x * 4u + (uint32_t)w; p < total; p += gridDim.x * 4u) {
+    const uint32_t sidx = uniform(find_segment(tab->prefix, nsegs, p));
+    const auto &sg = tab->seg[sidx];
+    if (tick >= sg.ticks) continue;
+    const bool odd = ((sg.parity + tick) & 1u) != 0;
+    char *base = static_cast<char *>(odd ? sg.f1 : sg.f0);
+    const uint32_t q = p - tab->prefix[sidx];
+    const uint32_t dst = sg.items[2 * q], src = sg.items[2 * q + 1];
+    const size_t step_bytes = (size_t)sg.step * (NV * 16 * 64 / 512);
+    const uint4 *sp = reinterpret_cast<const uint4 *>(base + (size_t)src * step_bytes);
+    uint4 *dp = reinterpret_cast<uint4 *>(base + (size_t)dst * step_bytes);
+    uint4 t[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t[v] = sp[lane + 64 * v];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) dp[lane + 64 * v] = t[v];
+  }
+}
+
+template <int NV>
+hipError_t launch_ghost_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  static int nblk = 0;
+  if (nblk == 0) {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    nblk = cus * 8;
+  }
+  hipLaunchKernelGGL(ghost_batched<NV>, dim3(nblk), dim3(256), 0, s, tab, tick);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // zero fill (a kernel, so it can sit inside a captured graph)
 // ---------------------------------------------------------------------------
@@ -301,12 +342,21 @@ hipError_t launch_ghost_refresh(void *dat, size_t step_bytes, size_t brick_bytes
   return hipGetLastError();
 }
 
+hipError_t launch_ghost_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  switch (dtype) {
+    case BK_F64: return launch_ghost_batched_t<4>(tab, tick, s);
+    case BK_F32: return launch_ghost_batched_t<2>(tab, tick, s);
+    case BK_BF16: return launch_ghost_batched_t<1>(tab, tick, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
   const size_t n = bytes / 16;
   if (n == 0) return hipSuccess;
   int dev = 0, cus = 256;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   size_t nblk = (n + 1023) / 1024;
   const size_t cap = (size_t)cus * 8;
   if (nblk > cap) nblk = cap;

@@ -20,9 +20,6 @@
 namespace bk {
 namespace {
 
-template <class A>
-using kconst_ptr = const A __attribute__((address_space(4))) *;
-
 template <class T>
 struct CubeArgs {
   using A = typename acc_of<T>::type;
@@ -115,23 +112,20 @@ __global__ __launch_bounds__(256) void cube_sweep_lds(const CubeArgs<T> a) {
 // points.  Per output the taps accumulate in (dk, dj, di) lexicographic
 // order, the first as a multiply -- the same arithmetic as the per-brick
 // kernel, so results are identical.
+constexpr int kQuadPW = 20, kQuadCells = kQuadPW * kQuadPW;
+
+// One plan unit of the quad schedule: members m0.. over L levels, store
+// masks mk[0..L), coefficients through the constant-address pointer cp.
+// plane: two LDS plane buffers, tbl: 16-entry LDS brick table; one wave.
 template <class T>
-__global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
+__device__ __forceinline__ void cube_quad_unit(
+    const uint32_t *__restrict__ adj, const T *__restrict__ in, T *__restrict__ out, size_t st,
+    size_t ost, uint32_t m0, int L, const uint32_t *__restrict__ mk,
+    kconst_ptr<typename acc_of<T>::type> cp0, typename acc_of<T>::type (*plane)[kQuadCells],
+    uint32_t *tbl, int lane) {
   using A = typename acc_of<T>::type;
-  constexpr int PW = 20, NCELL = PW * PW, NLD = (NCELL + 63) / 64;
-  __shared__ A plane[2][NCELL];
-  __shared__ uint32_t tbl[16];
-
-  const int lane = threadIdx.x;
+  constexpr int PW = kQuadPW, NCELL = kQuadCells, NLD = (NCELL + 63) / 64;
   const int lj = lane >> 3, li = lane & 7;
-  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
-  if (u >= a.nunits) return;
-  const uint32_t *__restrict__ adj = a.adj;
-  const int L = (int)a.len[u];
-  const uint32_t *mk = a.mask + (size_t)u * a.kcol;
-  const size_t st = a.in_step, ost = a.out_step;
-
-  const uint32_t m0 = a.anchor[u];
   const uint32_t m1 = adj[(size_t)m0 * 27 + 14];
   const uint32_t m2 = adj[(size_t)m0 * 27 + 16];
   const uint32_t m3 = adj[(size_t)m1 * 27 + 16];
@@ -168,7 +162,7 @@ __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
 
   A nv[NLD];
 #pragma unroll
-  for (int t = 0; t < NLD; ++t) nv[t] = cv[t] ? widen(a.in[base[t] + 6 * 64 + eoff[t]]) : A(0);
+  for (int t = 0; t < NLD; ++t) nv[t] = cv[t] ? widen(in[base[t] + 6 * 64 + eoff[t]]) : A(0);
 #pragma unroll
   for (int t = 0; t < NLD; ++t)
     if (cv[t]) plane[0][t * 64 + lane] = nv[t];
@@ -200,18 +194,16 @@ __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
       const int kp = Pn & 7;
 #pragma unroll
       for (int t = 0; t < NLD; ++t)
-        nv[t] = cv[t] ? widen(a.in[base[t] + kp * 64 + eoff[t]]) : A(0);
+        nv[t] = cv[t] ? widen(in[base[t] + kp * 64 + eoff[t]]) : A(0);
     }
 
     const A *pl = plane[buf];
     // Coefficients are read group by group (row r, slot s: ten values)
-    // straight from the kernel-argument segment through a constant-address
-    // pointer, so they arrive by scalar loads into SGPRs; the empty asm at
-    // each group hides the pointer's loop invariance so the compiler does
-    // not try to keep all 125 (250 SGPRs) live across the plane loop.  The
-    // argument struct is the kernel's only parameter and sits at offset 0.
-    kconst_ptr<A> cp = (kconst_ptr<A>)((kconst_ptr<char>)__builtin_amdgcn_kernarg_segment_ptr() +
-                                       offsetof(CubeArgs<T>, c));
+    // through a constant-address pointer, so they arrive by scalar loads
+    // into SGPRs; the empty asm at each group hides the pointer's loop
+    // invariance so the compiler does not try to keep all 125 (250 SGPRs)
+    // live across the plane loop.
+    kconst_ptr<A> cp = cp0;
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
       A x[6];
@@ -260,7 +252,7 @@ __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
       }
       if ((mk[lo] >> mm_l) & 1u) {
         const uint32_t ob = mm_l == 0 ? om0 : (mm_l == 1 ? om1 : (mm_l == 2 ? om2 : om3));
-        T *po = a.out + (size_t)ob * ost + kk * 64 + jo * 8 + io;
+        T *po = out + (size_t)ob * ost + kk * 64 + jo * 8 + io;
         narrow(po[0], acc[0][0][0]);
         narrow(po[1], acc[0][0][1]);
         narrow(po[8], acc[0][1][0]);
@@ -281,6 +273,61 @@ __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
     }
     __syncthreads();
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
+  using A = typename acc_of<T>::type;
+  __shared__ A plane[2][kQuadCells];
+  __shared__ uint32_t tbl[16];
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;
+  // The argument struct is the kernel's only parameter and sits at offset 0
+  // of the kernel-argument segment: its coefficients are read from there by
+  // scalar loads.
+  const kconst_ptr<A> cp =
+      (kconst_ptr<A>)((kconst_ptr<char>)__builtin_amdgcn_kernarg_segment_ptr() +
+                      offsetof(CubeArgs<T>, c));
+  cube_quad_unit<T>(a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], (int)a.len[u],
+                    a.mask + (size_t)u * a.kcol, cp, plane, tbl, threadIdx.x);
+}
+
+// Batched variant: one persistent launch serves every cube request of a
+// group table; work item = one quad-plan unit of one request.
+template <class T>
+__global__ __launch_bounds__(64) void cube_sweep_batched(const BatchTable *tab_, uint32_t tick) {
+  using A = typename acc_of<T>::type;
+  __shared__ A plane[2][kQuadCells];
+  __shared__ uint32_t tbl[16];
+  const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
+  const uint32_t nsegs = tab->nsegs;
+  const uint32_t total = tab->prefix[nsegs];
+  for (uint32_t u = xcd_remap(blockIdx.x, gridDim.x); u < total; u += gridDim.x) {
+    const uint32_t sidx = uniform(find_segment(tab->prefix, nsegs, u));
+    const auto &sg = tab->seg[sidx];
+    if (tick >= sg.ticks) continue;
+    const uint32_t ul = u - tab->prefix[sidx];
+    const bool odd = ((sg.parity + tick) & 1u) != 0;
+    const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
+    T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);
+    cube_quad_unit<T>(sg.adj, in, out, sg.step, sg.step, sg.items[ul], (int)sg.len[ul],
+                      sg.mask + (size_t)ul * sg.kcol, (kconst_ptr<A>)sg.coeff, plane, tbl,
+                      threadIdx.x);
+  }
+}
+
+template <class T>
+hipError_t launch_cube_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  static int nblk = 0;
+  if (nblk == 0) {
+    int dev = 0, cus = 256, per = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, cube_sweep_batched<T>, 64, 0);
+    nblk = cus * (per > 0 ? per : 1);
+  }
+  hipLaunchKernelGGL((cube_sweep_batched<T>), dim3(nblk), dim3(64), 0, s, tab, tick);
+  return hipGetLastError();
 }
 
 template <class T>
@@ -331,6 +378,15 @@ hipError_t launch_cube(const SweepLaunch &L, hipStream_t s) {
     case BK_F64: return launch_t<double>(L, s);
     case BK_F32: return launch_t<float>(L, s);
     case BK_BF16: return launch_t<bf16>(L, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_cube_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  switch (dtype) {
+    case BK_F64: return launch_cube_batched_t<double>(tab, tick, s);
+    case BK_F32: return launch_cube_batched_t<float>(tab, tick, s);
+    case BK_BF16: return launch_cube_batched_t<bf16>(tab, tick, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -71,4 +71,22 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Pointer into the constant address space: uniform loads through it are
+// scalar loads into SGPRs.  Used for coefficient tables and batch tables.
+template <class A>
+using kconst_ptr = const A __attribute__((address_space(4))) *;
+
+// Segment of a persistent batched launch holding `item`: the s with
+// prefix[s] <= item < prefix[s+1] (prefix has nsegs+1 entries).
+__device__ __forceinline__ uint32_t find_segment(kconst_ptr<uint32_t> prefix, uint32_t nsegs,
+                                                 uint32_t item) {
+  uint32_t lo = 0, hi = nsegs;  // invariant: prefix[lo] <= item < prefix[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace bk

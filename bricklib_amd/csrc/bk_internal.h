@@ -50,6 +50,43 @@ hipError_t launch_cube_quads(const SweepLaunch &L, const PlanView &P, hipStream_
 
 int plan_fail(int code, const char *msg);
 
+// ---- batched sweeps (bk_batch.hip) ---------------------------------------
+// A group table lists the requests (segments) one persistent kernel serves:
+// the concatenation of their work items (bricks for the stars, quad-plan
+// units for the cube, (dst, src) ghost pairs for the refresh).  Tables live
+// in HBM; the host rewrites them between runs, the kernels read them.
+constexpr uint32_t kBatchMaxSegs = 2048;
+
+struct BatchSeg {
+  const uint32_t *adj;    // the request's adjacency (its own brick numbering)
+  const uint32_t *items;  // stars: bricks; cube: plan anchors; refresh: pairs
+  const uint32_t *len;    // cube: levels per plan unit
+  const uint32_t *mask;   // cube: store masks [unit][kcol]
+  void *f0;               // the request's two fields
+  void *f1;
+  const void *coeff;      // coefficients in the accumulate type
+  uint64_t step;          // elements per brick, both fields
+  uint32_t nitems;
+  uint32_t kcol;
+  uint32_t parity;  // the field holding the input at tick 0 (0: f0)
+  uint32_t ticks;   // the request takes part in ticks [0, ticks) of a run
+  double theta;
+  int32_t cond_mode;
+  int32_t pad;
+};
+
+struct BatchTable {
+  uint32_t nsegs;
+  uint32_t pad;
+  uint32_t prefix[kBatchMaxSegs + 1];  // items before segment s
+  BatchSeg seg[kBatchMaxSegs];
+};
+
+hipError_t launch_star_batched(int stencil, int dtype, const BatchTable *tab, uint32_t tick,
+                               hipStream_t s);
+hipError_t launch_cube_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
+hipError_t launch_ghost_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
+
 hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,
                                 size_t brick_bytes, const uint32_t *pairs,
                                 uint32_t npairs, hipStream_t s);

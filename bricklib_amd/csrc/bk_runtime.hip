@@ -257,6 +257,23 @@ int bk_storage_alloc(int dtype, long chunks, size_t step, void **d_dat) {
   return bk_alloc(bytes, d_dat);
 }
 
+int bk_alloc_async(size_t bytes, void **d_ptr, bk_stream s) {
+  if (!d_ptr) return fail(BK_ERR_ARG, "bk_alloc_async: null out pointer");
+  *d_ptr = nullptr;
+  if (bytes == 0) return BK_OK;
+  hipError_t e = hipMallocAsync(d_ptr, bytes, S(s));
+  if (e == hipErrorOutOfMemory)
+    return fail(BK_ERR_NOMEM, "bk_alloc_async: %zu bytes: out of memory", bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+  return BK_OK;
+}
+
+int bk_free_async(void *d_ptr, bk_stream s) {
+  if (!d_ptr) return BK_OK;
+  BK_HIP(hipFreeAsync(d_ptr, S(s)));
+  return BK_OK;
+}
+
 int bk_free(void *d_ptr) {
   if (!d_ptr) return BK_OK;
   BK_HIP(hipFree(d_ptr));

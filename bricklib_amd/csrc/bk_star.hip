@@ -55,32 +55,28 @@ __device__ __forceinline__ A tapv(A x) {
   else return x;
 }
 
+// One brick of the per-brick schedule, in two phases around a workgroup
+// barrier: load() reads the brick's planes into registers and stages the
+// brick plus its i/j faces in this wave's LDS tile; compute() evaluates and
+// stores the 512 points.  Shared by the single-grid and the batched kernels.
 template <class T, int R, bool COND>
-__global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND> a) {
+struct StarBrick {
   using A = typename acc_of<T>::type;
-  constexpr int P = 8 + 2 * R;
-  __shared__ A tile[4][8][P][P];
-
-  const int w = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int j = lane >> 3, i = lane & 7;
-  const int fk = lane >> 3, fl = lane & 7;  // face-load lane mapping
-  const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * 4u + (uint32_t)w;
-  const bool active = slot < a.nlist;
-
+  static constexpr int P = 8 + 2 * R;
   A v[8], vkm[R], vkp[R];
-  uint32_t b = 0;
-  if (active) {
-    b = uniform(a.list[slot]);
-    const uint32_t *ab = a.adj + (size_t)b * 27;
-    const size_t st = a.in_step;
-    const T *pc = a.in + (size_t)b * st;
-    const T *pkm = a.in + (size_t)uniform(ab[4]) * st;
-    const T *pkp = a.in + (size_t)uniform(ab[22]) * st;
-    const T *pjm = a.in + (size_t)uniform(ab[10]) * st;
-    const T *pjp = a.in + (size_t)uniform(ab[16]) * st;
-    const T *pim = a.in + (size_t)uniform(ab[12]) * st;
-    const T *pip = a.in + (size_t)uniform(ab[14]) * st;
+
+  __device__ __forceinline__ void load(const uint32_t *__restrict__ adj, const T *__restrict__ in,
+                                       size_t st, uint32_t b, A (*tile)[P][P], int lane) {
+    const int j = lane >> 3, i = lane & 7;
+    const int fk = lane >> 3, fl = lane & 7;  // face-load lane mapping
+    const uint32_t *ab = adj + (size_t)b * 27;
+    const T *pc = in + (size_t)b * st;
+    const T *pkm = in + (size_t)uniform(ab[4]) * st;
+    const T *pkp = in + (size_t)uniform(ab[22]) * st;
+    const T *pjm = in + (size_t)uniform(ab[10]) * st;
+    const T *pjp = in + (size_t)uniform(ab[16]) * st;
+    const T *pim = in + (size_t)uniform(ab[12]) * st;
+    const T *pip = in + (size_t)uniform(ab[14]) * st;
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = widen(pc[k * 64 + lane]);
 #pragma unroll
@@ -97,40 +93,66 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
       fip[r] = widen(pip[fk * 64 + fl * 8 + r]);
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) tile[w][k][R + j][R + i] = v[k];
+    for (int k = 0; k < 8; ++k) tile[k][R + j][R + i] = v[k];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      tile[w][fk][r][R + fl] = fjm[r];
-      tile[w][fk][8 + R + r][R + fl] = fjp[r];
-      tile[w][fk][R + fl][r] = fim[r];
-      tile[w][fk][R + fl][8 + R + r] = fip[r];
+      tile[fk][r][R + fl] = fjm[r];
+      tile[fk][8 + R + r][R + fl] = fjp[r];
+      tile[fk][R + fl][r] = fim[r];
+      tile[fk][R + fl][8 + R + r] = fip[r];
     }
+  }
+
+  __device__ __forceinline__ void compute(const A (&c)[6 * R + 1], A theta, int cond_mode,
+                                          T *__restrict__ po, A (*tile)[P][P], int lane) {
+    const int j = lane >> 3, i = lane & 7;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      auto X = [&](int kk) -> A {
+        return kk < 0 ? vkm[kk + R] : (kk >= 8 ? vkp[kk - 8] : v[kk]);
+      };
+      A acc = c[0] * tapv<A, COND>(v[k]);
+#pragma unroll
+      for (int r = 1; r <= R; ++r) {
+        acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[k][R + j][R + i + r]), acc);
+        acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[k][R + j][R + i - r]), acc);
+        acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[k][R + j + r][R + i]), acc);
+        acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[k][R + j - r][R + i]), acc);
+        acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
+        acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
+      }
+      if constexpr (COND) {
+        const bool sel = cond_mode == 0 ? (acc > A(0)) : (v[k] > theta);
+        acc = sel ? acc : -acc;
+      }
+      store_nt(&po[k * 64 + lane], acc);
+    }
+  }
+};
+
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND> a) {
+  using A = typename acc_of<T>::type;
+  constexpr int P = 8 + 2 * R;
+  __shared__ A tile[4][8][P][P];
+
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * 4u + (uint32_t)w;
+  const bool active = slot < a.nlist;
+
+  StarBrick<T, R, COND> sb;
+  uint32_t b = 0;
+  if (active) {
+    b = uniform(a.list[slot]);
+    sb.load(a.adj, a.in, a.in_step, b, tile[w], lane);
   }
   __syncthreads();
   if (!active) return;
-
-  T *po = a.out + (size_t)b * a.out_step;
+  A c[6 * R + 1];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    auto X = [&](int kk) -> A {
-      return kk < 0 ? vkm[kk + R] : (kk >= 8 ? vkp[kk - 8] : v[kk]);
-    };
-    A acc = a.c[0] * tapv<A, COND>(v[k]);
-#pragma unroll
-    for (int r = 1; r <= R; ++r) {
-      acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[w][k][R + j][R + i + r]), acc);
-      acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[w][k][R + j][R + i - r]), acc);
-      acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[w][k][R + j + r][R + i]), acc);
-      acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[w][k][R + j - r][R + i]), acc);
-      acc = fmadd(a.c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
-      acc = fmadd(a.c[6 * r], tapv<A, COND>(X(k - r)), acc);
-    }
-    if constexpr (COND) {
-      const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (v[k] > a.theta);
-      acc = sel ? acc : -acc;
-    }
-    store_nt(&po[k * 64 + lane], acc);
-  }
+  for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
+  sb.compute(c, a.theta, a.cond_mode, a.out + (size_t)b * a.out_step, tile[w], lane);
 }
 
 
@@ -373,6 +395,80 @@ hipError_t launch_cols_dt(const SweepLaunch &L, const PlanView &P, hipStream_t s
   }
 }
 
+// Batched variant: one persistent launch serves every request of a group
+// table (same stencil and dtype).  Work item = one brick of one request;
+// the workgroup's four waves take four consecutive items per round, find
+// their request by binary search over the table's prefix counts, and run
+// the per-brick body with that request's adjacency, fields (ping-pong by
+// parity and tick) and coefficients.
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(256) void star_sweep_batched(const BatchTable *tab_, uint32_t tick) {
+  using A = typename acc_of<T>::type;
+  constexpr int P = 8 + 2 * R;
+  __shared__ A tile[4][8][P][P];
+  const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
+  const uint32_t nsegs = tab->nsegs;
+  const uint32_t total = tab->prefix[nsegs];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const uint32_t nwg = gridDim.x;
+  for (uint32_t base = xcd_remap(blockIdx.x, nwg) * 4u; base < total; base += nwg * 4u) {
+    const uint32_t item = base + (uint32_t)w;
+    bool active = item < total;
+    uint32_t sidx = 0, b = 0;
+    if (active) {
+      sidx = uniform(find_segment(tab->prefix, nsegs, item));
+      active = tick < tab->seg[sidx].ticks;
+    }
+    StarBrick<T, R, COND> sb;
+    const T *in = nullptr;
+    T *out = nullptr;
+    if (active) {
+      const auto &sg = tab->seg[sidx];
+      const bool odd = ((sg.parity + tick) & 1u) != 0;
+      in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
+      out = static_cast<T *>(odd ? sg.f0 : sg.f1);
+      b = uniform(sg.items[item - tab->prefix[sidx]]);
+      sb.load(sg.adj, in, sg.step, b, tile[w], lane);
+    }
+    __syncthreads();
+    if (active) {
+      const auto &sg = tab->seg[sidx];
+      const kconst_ptr<A> cp = (kconst_ptr<A>)sg.coeff;
+      A c[6 * R + 1];
+#pragma unroll
+      for (int q = 0; q < 6 * R + 1; ++q) c[q] = cp[q];
+      sb.compute(c, (A)sg.theta, sg.cond_mode, out + (size_t)b * sg.step, tile[w], lane);
+    }
+    __syncthreads();
+  }
+}
+
+template <class T, int R, bool COND>
+hipError_t launch_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  static int nblk = 0;
+  if (nblk == 0) {
+    int dev = 0, cus = 256, per = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, star_sweep_batched<T, R, COND>, 256, 0);
+    nblk = cus * (per > 0 ? per : 1);
+  }
+  hipLaunchKernelGGL((star_sweep_batched<T, R, COND>), dim3(nblk), dim3(256), 0, s, tab, tick);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_batched_dt(int stencil, const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  switch (stencil) {
+    case BK_STAR7: return launch_batched_t<T, 1, false>(tab, tick, s);
+    case BK_STAR13: return launch_batched_t<T, 2, false>(tab, tick, s);
+    case BK_STAR25: return launch_batched_t<T, 4, false>(tab, tick, s);
+    case BK_COND7: return launch_batched_t<T, 1, true>(tab, tick, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <class T, int R, bool COND>
 hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   using A = typename acc_of<T>::type;
@@ -412,6 +508,16 @@ hipError_t launch_star(const SweepLaunch &L, hipStream_t s) {
     case BK_F64: return launch_dt<double>(L, s);
     case BK_F32: return launch_dt<float>(L, s);
     case BK_BF16: return launch_dt<bf16>(L, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_star_batched(int stencil, int dtype, const BatchTable *tab, uint32_t tick,
+                               hipStream_t s) {
+  switch (dtype) {
+    case BK_F64: return launch_batched_dt<double>(stencil, tab, tick, s);
+    case BK_F32: return launch_batched_dt<float>(stencil, tab, tick, s);
+    case BK_BF16: return launch_batched_dt<bf16>(stencil, tab, tick, s);
     default: return hipErrorInvalidValue;
   }
 }

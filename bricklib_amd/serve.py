@@ -1,0 +1,150 @@
+"""Serving loop: continuous batching of stencil solves on one GPU.
+
+Requests (grid size, stencil, dtype, sweep count, seed) arrive over time.
+The loop admits every arrived request whose fields fit the HBM budget
+(fields stay resident until the request completes), runs all admitted
+requests together for a chunk of ticks in one batched run, retires the
+completed ones (their memory returns to the budget) and admits new arrivals
+between chunks -- requests that arrive while others are on the GPU join the
+next chunk, so their work overlaps on the device.  Latency of a request =
+completion time - arrival time; throughput = interior cells x sweeps of all
+completed requests / elapsed time.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import BK_F32, BK_F64
+from .batch import Batch, Request, ShapeCache
+from .bricks import Event, Stream
+from .stencils import NAMES, Stencil, u01
+
+
+def workload(n: int, rate: float, seed: int = 2024, edges=(32, 256), max_steps: int = 200,
+             stencils=("7pt", "13pt", "25pt", "125pt"), dtypes=(BK_F64, BK_F32)) -> list:
+    """Poisson arrivals at `rate` per second; each edge uniform over the
+    multiples of 8 in [edges[0], edges[1]] (non-cubic), stencil uniform over
+    `stencils`, dtype uniform over `dtypes`, steps uniform in [1, max_steps],
+    per-request seed.  Deterministic from `seed`."""
+    out = []
+    t = 0.0
+    choices = np.arange(edges[0], edges[1] + 1, 8)
+    for r in range(n):
+        h = lambda q: u01(seed, 16 * r + q)  # noqa: E731
+        t += -np.log(1.0 - h(0)) / rate if rate > 0 else 0.0
+        nb = tuple(int(choices[min(int(h(1 + a) * len(choices)), len(choices) - 1)]) // 8
+                   for a in range(3))
+        sname = stencils[min(int(h(4) * len(stencils)), len(stencils) - 1)]
+        dt = dtypes[min(int(h(5) * len(dtypes)), len(dtypes) - 1)]
+        steps = 1 + min(int(h(6) * max_steps), max_steps - 1)
+        rseed = int(h(7) * (1 << 62))
+        out.append(Request(rid=r, nb=nb, stencil=Stencil.make(sname, seed=rseed), dtype=dt,
+                           steps=steps, seed=rseed, arrival=t))
+    return out
+
+
+@dataclass
+class ServeStats:
+    completed: list = field(default_factory=list)
+    elapsed: float = 0.0
+    peak_bytes: int = 0
+    chunks: int = 0
+
+    def latencies(self) -> np.ndarray:
+        return np.array([r.finished - r.arrival for r in self.completed])
+
+    def summary(self) -> dict:
+        lat = self.latencies()
+        work = sum(r.cells * r.steps for r in self.completed)
+        return {"requests": len(self.completed),
+                "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
+                "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
+                "mean_latency_s": float(lat.mean()) if lat.size else None,
+                "elapsed_s": self.elapsed,
+                "gstencil_s": work / self.elapsed / 1e9 if self.elapsed > 0 else 0.0,
+                "stencil_updates": work, "peak_hbm_bytes": self.peak_bytes,
+                "chunks": self.chunks}
+
+
+class Server:
+    """Continuous-batching loop over one Batch."""
+
+    def __init__(self, hbm_budget: float = 200e9, chunk: int = 8, max_active: int = 2048,
+                 use_graph: bool = True):
+        self.stream = Stream()
+        self.batch = Batch(max_active, stream=self.stream)
+        self.shapes = ShapeCache()
+        self.budget = float(hbm_budget)
+        self.chunk = int(chunk)
+        self.max_active = int(max_active)
+        self.use_graph = use_graph
+
+    def admissible(self, r: Request, used: int, nactive: int) -> bool:
+        return nactive < self.max_active and used + r.hbm_bytes <= self.budget
+
+    def serve(self, requests: list, realtime: bool = True, keep_results: bool = False,
+              on_complete=None) -> ServeStats:
+        """Serve `requests` (sorted by arrival).  realtime: arrivals follow
+        the wall clock from the start; otherwise all are present at t=0
+        (a saturating, closed-loop run)."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        if not realtime:
+            for r in pending:
+                r.arrival = 0.0
+        for r in pending:
+            if r.hbm_bytes > self.budget:
+                raise ValueError(f"request {r.rid} needs {r.hbm_bytes} B, above the budget")
+        active: list = []
+        used = 0
+        st = ServeStats()
+        t0 = time.perf_counter()
+        qi = 0
+        ev = Event()
+        while qi < len(pending) or active:
+            now = time.perf_counter() - t0
+            # admit arrived requests that fit
+            while qi < len(pending) and pending[qi].arrival <= now:
+                r = pending[qi]
+                if not self.admissible(r, used, len(active)):
+                    break
+                r.allocate(self.shapes, self.stream)
+                r.admitted = now
+                used += r.hbm_bytes
+                active.append(r)
+                qi += 1
+            st.peak_bytes = max(st.peak_bytes, used)
+            if not active:
+                if qi < len(pending):
+                    time.sleep(max(0.0, pending[qi].arrival - (time.perf_counter() - t0)))
+                continue
+            ticks = min(self.chunk, max(r.remaining for r in active))
+            self.batch.run(active, ticks, self.use_graph)
+            ev.record(self.stream)
+            ev.sync()
+            st.chunks += 1
+            now = time.perf_counter() - t0
+            still = []
+            for r in active:
+                if r.remaining == 0:
+                    r.finished = now
+                    if on_complete is not None:
+                        on_complete(r)
+                    if not keep_results:
+                        r.release()
+                    used -= r.hbm_bytes
+                    st.completed.append(r)
+                else:
+                    still.append(r)
+            active = still
+        st.elapsed = time.perf_counter() - t0
+        return st
+
+
+def stencil_name(kind: int) -> str:
+    for k, v in NAMES.items():
+        if v == kind:
+            return k
+    return str(kind)

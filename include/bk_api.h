@@ -74,6 +74,7 @@ enum bk_cond_mode { BK_COND_REFERENCE = 0, BK_COND_THRESHOLD = 1 };
 typedef void *bk_stream; /* hipStream_t */
 typedef void *bk_event;  /* hipEvent_t */
 typedef struct bk_plan_s *bk_plan; /* execution plan (bk_plan_create) */
+typedef struct bk_batch_s *bk_batch; /* batched sweeps (bk_batch_create) */
 
 enum bk_plan_shape { BK_PLAN_COLUMN = 1, BK_PLAN_QUAD = 4 };
 
@@ -96,6 +97,11 @@ int bk_device_sync(void);
 int bk_storage_alloc(int dtype, long chunks, size_t step, void **d_dat);
 int bk_free(void *d_ptr);
 int bk_alloc(size_t bytes, void **d_ptr);
+/* Stream-ordered allocation and release (hipMallocAsync / hipFreeAsync):
+ * no device-wide synchronisation, for loops that admit and retire grids
+ * while other grids are being swept. */
+int bk_alloc_async(size_t bytes, void **d_ptr, bk_stream s);
+int bk_free_async(void *d_ptr, bk_stream s);
 /* Host<->device copies: movBrickStorage (brick-gpu.h:84-103) and
  * movBrickInfo (brick-gpu.h:42-55) are built on these.  Sizes are size_t
  * (the reference computes them in `unsigned`, brick-gpu.h:47,89). */
@@ -226,6 +232,52 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs,
               int nsweeps, int use_graph, bk_stream s);
 /* Drop every cached graph (frees their device resources). */
 int bk_graph_cache_clear(void);
+
+/* ---- batched sweeps over many grids ------------------------------------ */
+/* One request of a batch: an independent grid with its own fields,
+ * adjacency, stencil, dtype and coefficients.  A batch runs ticks; at tick t
+ * every request with t < ticks refreshes the ghost bricks of its input
+ * field (pairs from bk_ghost_pairs, may be none) and sweeps it into the
+ * other field: the input is f0 when (parity + t) is even, f1 otherwise.
+ * Stars sweep the bricks of d_list; the cube sweeps the bricks of `plan`,
+ * which must be a BK_PLAN_QUAD plan.  This is the weak-scaling loop of
+ * weak/main.hip.cpp:243-284 for many subdomains at once, the way the
+ * strong-scaling driver's one launch over many subdomains does
+ * (strong/main.cu:85-99), with grids of any size, stencil and dtype. */
+typedef struct bk_batch_req {
+  int stencil;
+  int dtype;
+  const uint32_t *d_adj;
+  const uint32_t *d_list; /* stars: bricks to compute */
+  uint32_t nlist;
+  bk_plan plan;           /* cube: BK_PLAN_QUAD plan of the bricks */
+  const uint32_t *d_pairs; /* ghost (dst, src) pairs, or NULL */
+  uint32_t npairs;
+  void *d_f0;
+  void *d_f1;
+  size_t step;            /* elements per brick in both fields */
+  const double *h_coeff;  /* copied by bk_batch_set */
+  double theta;
+  int cond_mode;
+  uint32_t parity;        /* 0: f0 holds the input at tick 0 */
+  uint32_t ticks;         /* ticks of the next bk_batch_run it takes part in */
+} bk_batch_req;
+
+/* A batch serving up to max_reqs requests (<= 2048): one persistent kernel
+ * per (stencil, dtype) group and per tick, each over the concatenated work
+ * of all requests in the group, plus one persistent refresh kernel per
+ * dtype.  Allocates its tables (HBM and pinned host memory). */
+int bk_batch_create(uint32_t max_reqs, bk_batch *batch);
+int bk_batch_destroy(bk_batch batch);
+/* Replace the batch's requests (stream-ordered copies of the tables and
+ * coefficients; the host arrays may be reused when the call returns).  The
+ * device pointers in the requests must stay valid until the runs that use
+ * them have completed. */
+int bk_batch_set(bk_batch batch, const bk_batch_req *reqs, uint32_t n, bk_stream s);
+/* Run nticks ticks.  With use_graph != 0 the launches are captured once per
+ * (nticks, set of non-empty groups) and replayed; the tables are read when
+ * the kernels run, so later bk_batch_set calls need no new capture. */
+int bk_batch_run(bk_batch batch, uint32_t nticks, int use_graph, bk_stream s);
 
 /* ---- seeded input and FP64 reductions ---------------------------------- */
 /* Fill the interior of a padded brick grid (nb interior bricks per axis,

@@ -1,0 +1,109 @@
+"""Batched sweeps and the serving loop on the GPU.
+
+Each request's result is the same alone and in the mix: bit for bit against
+the single-grid path (and, for FP64, the oracle), whatever the mix, chunk
+size or admission order; requests that do not fit the HBM budget wait and
+still complete; malformed requests are rejected."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bricklib_amd import BK_BF16, BK_CUBE125, BK_F32, BK_F64, serve
+from bricklib_amd.batch import Batch, Request, ShapeCache
+from bricklib_amd.stencils import Stencil
+
+pytestmark = pytest.mark.gpu
+
+MIX = [((2, 3, 4), "7pt", BK_F64, 5), ((4, 4, 4), "125pt", BK_F64, 3),
+       ((3, 2, 2), "13pt", BK_F32, 7), ((2, 2, 5), "25pt", BK_F64, 2),
+       ((5, 3, 1), "cond", BK_F64, 4), ((2, 4, 2), "125pt", BK_F32, 6),
+       ((1, 1, 1), "7pt", BK_BF16, 9), ((4, 2, 3), "25pt", BK_F32, 1)]
+
+
+def make_mix():
+    reqs = []
+    for q, (nb, name, dt, steps) in enumerate(MIX):
+        reqs.append(Request(rid=q, nb=nb, stencil=Stencil.make(name, seed=100 + q), dtype=dt,
+                            steps=steps, seed=1000 + q))
+    return reqs
+
+
+def alone(gpu, r: Request) -> np.ndarray:
+    """The same solve through the single-grid path."""
+    g = gpu.PeriodicGrid(r.nb)
+    a = gpu.Brick(g.info, g.field_storage(r.dtype), 0)
+    b = gpu.Brick(g.info, g.field_storage(r.dtype), 0)
+    g.fill_uniform(a, r.seed)
+    res = gpu.sweeps(r.stencil, a, b, g, r.steps)
+    return res.storage.to_host(), g.interior
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 8])
+def test_mix_matches_alone(gpu, chunk):
+    srv = serve.Server(chunk=chunk)
+    reqs = make_mix()
+    stats = srv.serve(reqs, realtime=False, keep_results=True)
+    assert len(stats.completed) == len(reqs)
+    for r in reqs:
+        ref, interior = alone(gpu, r)
+        got = r.result_host()
+        print(f"[case] rid={r.rid} {r.nb} kind={r.stencil.kind} dtype={r.dtype} steps={r.steps}")
+        np.testing.assert_array_equal(got[interior], ref[interior])
+
+
+def test_mix_fp64_matches_oracle(gpu):
+    srv = serve.Server(chunk=4)
+    reqs = [r for r in make_mix() if r.dtype == BK_F64]
+    srv.serve(reqs, realtime=False, keep_results=True)
+    for r in reqs:
+        og = O.Grid(r.nb)
+        oa, ob = og.storage(), og.storage()
+        og.fill_uniform(oa, r.seed)
+        ores = og.sweeps(r.stencil.kind, r.stencil.coeff, oa, ob, r.steps,
+                         theta=r.stencil.theta, cond_mode=r.stencil.cond_mode)
+        np.testing.assert_array_equal(r.result_host()[og.interior], ores[og.interior])
+
+
+def test_tight_budget_queues_and_completes(gpu):
+    reqs = make_mix()
+    budget = max(r.hbm_bytes for r in reqs) * 2  # about two requests at a time
+    srv = serve.Server(hbm_budget=budget, chunk=2)
+    stats = srv.serve(reqs, realtime=False, keep_results=True)
+    assert len(stats.completed) == len(reqs)
+    assert stats.peak_bytes <= budget
+    for r in reqs[:4]:
+        ref, interior = alone(gpu, r)
+        np.testing.assert_array_equal(r.result_host()[interior], ref[interior])
+
+
+def test_poisson_stream_small(gpu):
+    reqs = serve.workload(24, rate=400.0, edges=(16, 48), max_steps=12, seed=5)
+    srv = serve.Server(chunk=4)
+    stats = srv.serve(reqs, realtime=True)
+    s = stats.summary()
+    print(s)
+    assert s["requests"] == 24 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+
+
+def test_malformed_requests_rejected(gpu):
+    from bricklib_amd import BrickError
+    shapes = ShapeCache()
+    r = Request(rid=0, nb=(2, 2, 2), stencil=Stencil.make("125pt"), dtype=BK_F64, steps=1,
+                seed=1)
+    r.allocate(shapes)
+    b = Batch(4)
+    rq = r.req_struct(1)
+    rq.plan = None  # a cube request without its quad plan
+    from bricklib_amd._lib import bk_batch_req, load
+    arr = (bk_batch_req * 1)(rq)
+    assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 1
+    rq = r.req_struct(1)
+    rq.d_f1 = rq.d_f0  # aliased fields
+    arr = (bk_batch_req * 1)(rq)
+    assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 1
+    with pytest.raises(BrickError):
+        Batch(0)
+    with pytest.raises(ValueError):
+        serve.Server(hbm_budget=1000).serve([r], realtime=False)
