@@ -19,7 +19,13 @@ The same JSON line also carries:
                   plus the FP32 / BF16 forms of the 7-point star;
   precision    -- FP32 and BF16-storage forms against the FP64 sweep after
                   1, 10 and 100 sweeps at 512^3 and 1024^3 (errors computed
-                  in FP64 on the GPU) and their speed-up over FP64.
+                  in FP64 on the GPU) and their speed-up over FP64;
+  serving      -- continuous batching of independent solves (edges 32..256,
+                  7/13/25/125-point, FP64/FP32, 1..200 sweeps): Poisson
+                  arrivals at 200 req/s (p50/p99 latency), a saturating run
+                  (all requests present at t=0) against the single-grid
+                  512^3 rates, and a sample's errors against the same solve
+                  alone in FP64.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -52,6 +58,9 @@ def parse():
     p.add_argument("--only", default="", help="comma list of extra cases to run (default all)")
     p.add_argument("--no-extra", action="store_true")
     p.add_argument("--no-precision", action="store_true")
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--serve-requests", type=int, default=600)
+    p.add_argument("--serve-rate", type=float, default=200.0)
     p.add_argument("--precision-1024", action="store_true", default=True,
                    help="also run the precision study at twice the size (1024^3)")
     return p.parse_args()
@@ -147,6 +156,88 @@ def cpu_baseline(nb, coeff, seconds):
             "cores": int(O.lib().or_num_threads()), "kind": "port",
             "sample": f"{n} sweeps (periodic refresh + 7-pt brick loop) over the full "
                       f"{8 * nb[0]}^3 FP64 grid, {el:.1f} s"}
+
+
+def serve_bench(bk, S, args):
+    """Batched serving: Poisson arrivals at --serve-rate (p50/p99 latency),
+    then the same kind of stream all present at t=0 (saturating) for the
+    throughput, compared with the single-grid rates of each (stencil,
+    dtype) at 512^3; per-request errors of a sample against FP64 solves of
+    the same requests run alone (the FP64 GPU oracle)."""
+    from bricklib_amd import BK_F32, BK_F64, precision as PR, serve as SV
+    from bricklib_amd.batch import Request
+    out = {}
+    # single-grid reference rates at 512^3 (refresh + sweep, default schedule)
+    g = bk.PeriodicGrid((64, 64, 64))
+    stream = bk.Stream()
+    rates = {}
+    for name in ("7pt", "13pt", "25pt", "125pt"):
+        for dt in (BK_F64, BK_F32):
+            st = S.Stencil.make(name, seed=3)
+            fa = bk.Brick(g.info, g.field_storage(dt), 0)
+            fb = bk.Brick(g.info, g.field_storage(dt), 0)
+            g.fill_uniform(fa, 3, stream=stream)
+            ms = time_sweeps(bk, st, fa, fb, g, stream, 10, 2)
+            rates[(S.NAMES[name], dt)] = g.cells / (ms * 1e-3)
+            del fa, fb
+    del g
+    out["single_grid_gstencil_s"] = {f"{SV.stencil_name(k)}_{PR.DTYPE_NAMES[d]}": round(v / 1e9, 2)
+                                     for (k, d), v in rates.items()}
+
+    # real-time Poisson stream
+    reqs = SV.workload(args.serve_requests, rate=args.serve_rate, seed=2024)
+    srv = SV.Server(hbm_budget=200e9, chunk=8)
+    srv.prewarm(reqs)
+    sample = set(r.rid for r in reqs[::max(1, len(reqs) // 12)])
+    st = srv.serve(reqs, realtime=True, keep_results=sample)
+    rt = st.summary()
+    rt["rate_req_s"] = args.serve_rate
+    rt["offered_gstencil_s"] = round(sum(r.cells * r.steps for r in reqs) /
+                                     max(reqs[-1].arrival, 1e-9) / 1e9, 2)
+    out["poisson"] = rt
+    print(f"[bench] serving poisson: {rt}", file=sys.stderr, flush=True)
+
+    # per-request error of the sample against the same request alone in FP64
+    errs = []
+    for r in reqs:
+        if r.rid not in sample:
+            continue
+        ref = Request(rid=-1, nb=r.nb, stencil=r.stencil, dtype=BK_F64, steps=r.steps,
+                      seed=r.seed)
+        srv.serve([ref], realtime=False, keep_results=True)
+        gr = r.shape.grid
+        a = bk.Brick(gr.info, bk.BrickStorage(_View(r.result_ptr, r.field_bytes), gr.nbricks,
+                                              512, r.dtype), 0)
+        b = bk.Brick(gr.info, bk.BrickStorage(_View(ref.result_ptr, ref.field_bytes),
+                                              gr.nbricks, 512, BK_F64), 0)
+        c = PR.compare(gr, a, b)
+        errs.append({"rid": r.rid, "stencil": SV.stencil_name(r.stencil.kind),
+                     "dtype": PR.DTYPE_NAMES[r.dtype], "steps": r.steps, "nb": list(r.nb),
+                     "max_rel": c["max_rel"], "rms_rel": c["rms_rel"],
+                     "underflow": c["underflow"]})
+        r.release()
+        ref.release()
+    out["sample_errors_vs_fp64"] = errs
+
+    # saturating run: the same mix with every request present at t=0
+    reqs2 = SV.workload(args.serve_requests // 2, rate=args.serve_rate, seed=77)
+    srv.prewarm(reqs2)
+    st2 = srv.serve(reqs2, realtime=False)
+    sat = st2.summary()
+    single_time = sum(r.cells * r.steps / rates[(r.stencil.kind, r.dtype)] for r in reqs2)
+    sat["single_grid_time_s"] = round(single_time, 4)
+    sat["efficiency_vs_single_grid"] = round(single_time / st2.elapsed, 4)
+    out["saturating"] = sat
+    print(f"[bench] serving saturating: {sat}", file=sys.stderr, flush=True)
+    return out
+
+
+class _View:
+    """A non-owning view of device memory that a Request owns."""
+
+    def __init__(self, ptr, nbytes):
+        self.ptr = ptr
+        self.nbytes = nbytes
 
 
 def main():
@@ -253,6 +344,10 @@ def main():
                 del gp
                 print(f"[bench] precision {key}: {prec[key]}", file=sys.stderr, flush=True)
 
+    serving = {}
+    if not args.no_serve:
+        serving = serve_bench(bk, S, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nb, st7.coeff, args.cpu_seconds)
@@ -285,6 +380,7 @@ def main():
         "cpu_baseline": cpu,
         "by_stencil": by,
         "precision": prec,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -104,6 +104,14 @@ SIGNATURES = {
     "bk_graph_cache_clear": (C.c_int, []),
     "bk_fill_uniform": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_uint64, C.c_double,
                                   C.c_double, vp]),
+    "bk_arena_create": (C.c_int, [C.c_size_t, C.c_size_t, C.POINTER(vp)]),
+    "bk_arena_destroy": (C.c_int, [vp]),
+    "bk_arena_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
+    "bk_arena_free": (C.c_int, [vp, vp]),
+    "bk_arena_stats": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                 C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
+    "bk_fill_uniform_grid": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_int, C.c_uint64,
+                                       C.c_double, C.c_double, vp]),
     "bk_field_stats": (C.c_int, [C.c_int, vp, C.c_size_t, vp, C.c_uint32, C.c_size_t, dblp, vp]),
     "bk_field_compare": (C.c_int, [C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_size_t, vp,
                                    C.c_uint32, C.c_size_t, C.c_double, C.c_double, C.c_double,

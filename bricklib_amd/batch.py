@@ -22,13 +22,41 @@ from .bricks import DTYPE_SIZE, DeviceBuffer, PeriodicGrid, _stream_of, _vp
 from .stencils import Stencil
 
 
+def _record_dtype():
+    """A numpy view of bk_batch_req (field offsets from the ctypes layout;
+    pointers as uint64), so the per-run fields of many requests can be
+    written at once."""
+    names, formats, offsets = [], [], []
+    for fname, ftype in bk_batch_req._fields_:
+        names.append(fname)
+        offsets.append(getattr(bk_batch_req, fname).offset)
+        if ftype in (C.c_int, C.c_int32):
+            formats.append(np.int32)
+        elif ftype in (C.c_uint32,):
+            formats.append(np.uint32)
+        elif ftype in (C.c_double,):
+            formats.append(np.float64)
+        elif ftype in (C.c_size_t, C.c_uint64):
+            formats.append(np.uint64)
+        else:  # pointers and opaque handles
+            formats.append(np.uint64)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets,
+                     "itemsize": C.sizeof(bk_batch_req)})
+
+
+REQ_DTYPE = _record_dtype()
+
+
 class Shape:
     """Shared per-shape device tables (adjacency, interior list, ghost
-    pairs, quad plan)."""
+    pairs, quad plan).  ghost=0: the wrapped-adjacency layout (no ghost
+    bricks, no refresh); ghost=1: the reference's one-brick ghost zone with
+    a periodic refresh before every sweep."""
 
-    def __init__(self, nb, stream=None, kcol: int = 8):
-        self.grid = PeriodicGrid(nb, stream=stream)
+    def __init__(self, nb, stream=None, kcol: int = 8, ghost: int = 0):
+        self.grid = PeriodicGrid(nb, stream=stream, ghost=ghost)
         self.nb = self.grid.nb
+        self.ghost = self.grid.ghost
         self.quad = self.grid.plan("quad", kcol)
 
     @property
@@ -45,10 +73,10 @@ class ShapeCache:
         self.stream = stream
         self.shapes: dict = {}
 
-    def get(self, nb) -> Shape:
-        key = tuple(int(x) for x in nb)
+    def get(self, nb, ghost: int = 0) -> Shape:
+        key = (tuple(int(x) for x in nb), int(ghost))
         if key not in self.shapes:
-            self.shapes[key] = Shape(key, self.stream)
+            self.shapes[key] = Shape(key[0], self.stream, ghost=key[1])
         return self.shapes[key]
 
 
@@ -74,6 +102,60 @@ class AsyncBuffer:
             pass
 
 
+class Arena:
+    """Field arena (bk_arena_*): device blocks carved best-fit out of
+    hipMalloc'd slabs and recycled without a driver call.  Blocks must be
+    used on one stream (a freed block can be handed out again while queued
+    work still reads it)."""
+
+    def __init__(self, slab_bytes: int = 1 << 30, capacity: int = 0):
+        h = C.c_void_p()
+        check(load().bk_arena_create(C.c_size_t(int(slab_bytes)), C.c_size_t(int(capacity)),
+                                     C.byref(h)))
+        self.handle = h.value
+
+    def alloc(self, nbytes: int) -> "ArenaBlock":
+        return ArenaBlock(self, nbytes)
+
+    def stats(self) -> dict:
+        u, r, lf = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        n = C.c_uint32()
+        check(load().bk_arena_stats(_vp(self.handle), C.byref(u), C.byref(r), C.byref(lf),
+                                    C.byref(n)))
+        return {"used": u.value, "reserved": r.value, "largest_free": lf.value,
+                "live_blocks": n.value}
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().bk_arena_destroy(_vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
+class ArenaBlock:
+    """One block of an Arena (keeps the arena alive)."""
+
+    def __init__(self, arena: Arena, nbytes: int):
+        p = C.c_void_p()
+        check(load().bk_arena_alloc(_vp(arena.handle), C.c_size_t(int(nbytes)), C.byref(p)))
+        self.arena = arena
+        self.ptr = p.value or 0
+        self.nbytes = int(nbytes)
+
+    def free(self):
+        if self.ptr:
+            load().bk_arena_free(_vp(self.arena.handle), _vp(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 @dataclass
 class Request:
     """One independent stencil solve: nb interior bricks per axis, `steps`
@@ -86,15 +168,17 @@ class Request:
     steps: int
     seed: int
     arrival: float = 0.0
+    ghost: int = 0  # field layout: 0 wrapped adjacency, 1 ghost zone + refresh
     # runtime state
     shape: Shape | None = None
-    f0: AsyncBuffer | None = None
-    f1: AsyncBuffer | None = None
+    f0: object = None  # AsyncBuffer or ArenaBlock
+    f1: object = None
     parity: int = 0
     done: int = 0
     admitted: float = 0.0
     finished: float = 0.0
     _coeff: np.ndarray = field(default=None, repr=False)
+    _rec: bytes = field(default=None, repr=False)
 
     @property
     def cells(self) -> int:
@@ -102,7 +186,7 @@ class Request:
 
     @property
     def field_bytes(self) -> int:
-        return int(np.prod([x + 2 for x in self.nb])) * 512 * DTYPE_SIZE[self.dtype]
+        return int(np.prod([x + 2 * self.ghost for x in self.nb])) * 512 * DTYPE_SIZE[self.dtype]
 
     @property
     def hbm_bytes(self) -> int:
@@ -112,23 +196,31 @@ class Request:
     def remaining(self) -> int:
         return self.steps - self.done
 
-    def allocate(self, shapes: ShapeCache, stream=None) -> None:
-        """Fields in HBM (stream-ordered), input filled from the seed."""
-        self.shape = shapes.get(self.nb)
-        self.f0 = AsyncBuffer(self.field_bytes, stream)
-        self.f1 = AsyncBuffer(self.field_bytes, stream)
-        check(load().bk_fill_uniform(self.dtype, _vp(self.f0.ptr), C.c_size_t(512),
-                                     _lib.longs(self.nb), C.c_uint64(self.seed), -1.0, 1.0,
-                                     _vp(_stream_of(stream))))
+    def allocate(self, shapes: ShapeCache, stream=None, arena: Arena | None = None) -> None:
+        """Fields in HBM -- from `arena` when given (then all of this
+        request's work must run on `stream`), else stream-ordered -- and the
+        input filled from the seed."""
+        self.shape = shapes.get(self.nb, self.ghost)
+        if arena is not None:
+            self.f0 = arena.alloc(self.field_bytes)
+            self.f1 = arena.alloc(self.field_bytes)
+        else:
+            self.f0 = AsyncBuffer(self.field_bytes, stream)
+            self.f1 = AsyncBuffer(self.field_bytes, stream)
+        check(load().bk_fill_uniform_grid(self.dtype, _vp(self.f0.ptr), C.c_size_t(512),
+                                          _lib.longs(self.nb), self.ghost, C.c_uint64(self.seed),
+                                          -1.0, 1.0, _vp(_stream_of(stream))))
         self.parity = 0
         self.done = 0
         self._coeff = np.ascontiguousarray(self.stencil.coeff, np.float64)
+        self._rec = bytes(self.req_struct(0))
 
     def release(self) -> None:
         for f in (self.f0, self.f1):
             if f is not None:
                 f.free()
         self.f0 = self.f1 = None
+        self._rec = None
 
     @property
     def result_ptr(self) -> int:
@@ -141,6 +233,11 @@ class Request:
         check(load().bk_memcpy_d2h(out.ctypes.data_as(C.c_void_p), _vp(self.result_ptr),
                                    C.c_size_t(out.nbytes), _vp(_stream_of(stream))))
         return out.reshape(-1, 512)
+
+    def interior_host(self, stream=None) -> np.ndarray:
+        """The latest result's interior bricks in grid order (k slowest), the
+        same array for either layout."""
+        return self.result_host(stream)[self.shape.grid.interior]
 
     def req_struct(self, ticks: int) -> bk_batch_req:
         g = self.shape.grid
@@ -175,19 +272,25 @@ class Batch:
         self.stream = stream
 
     def run(self, reqs: list, ticks: int, use_graph: bool = True) -> None:
-        """`ticks` ticks: each request sweeps min(ticks, remaining) times."""
-        if len(reqs) > self.max_reqs:
+        """`ticks` ticks: each request sweeps min(ticks, remaining) times.
+        The requests' records are built once at allocation; per run only
+        their tick counts and parities are written (vectorised)."""
+        n = len(reqs)
+        if n > self.max_reqs:
             raise ValueError("more requests than the batch holds")
-        arr = (bk_batch_req * max(1, len(reqs)))()
-        takes = []
-        for q, r in enumerate(reqs):
-            arr[q] = r.req_struct(ticks)
-            takes.append(arr[q].ticks)
         st = _vp(_stream_of(self.stream))
-        check(load().bk_batch_set(_vp(self.handle), arr, len(reqs), st))
-        if reqs:
-            check(load().bk_batch_run(_vp(self.handle), int(ticks), int(bool(use_graph)), st))
-        for r, t in zip(reqs, takes):
+        if n == 0:
+            check(load().bk_batch_set(_vp(self.handle), None, 0, st))
+            return
+        recs = np.frombuffer(bytearray(b"".join([r._rec for r in reqs])), dtype=REQ_DTYPE)
+        rem = np.fromiter((r.steps - r.done for r in reqs), np.int64, n)
+        take = np.minimum(rem, int(ticks)).astype(np.uint32)
+        recs["ticks"] = take
+        recs["parity"] = np.fromiter((r.parity for r in reqs), np.uint32, n)
+        check(load().bk_batch_set(_vp(self.handle), recs.ctypes.data_as(C.POINTER(bk_batch_req)),
+                                  n, st))
+        check(load().bk_batch_run(_vp(self.handle), int(ticks), int(bool(use_graph)), st))
+        for r, t in zip(reqs, take.tolist()):
             r.done += t
             r.parity ^= t & 1
 

@@ -331,26 +331,41 @@ class Plan:
 # periodic brick grid with one ghost layer
 # ---------------------------------------------------------------------------
 class PeriodicGrid:
-    """n = 8*nb interior cells per axis plus an 8-cell (one-brick) ghost
-    zone: a (nb+2)^3 brick grid numbered by init_grid, its adjacency, the
-    interior brick list and the periodic ghost pairs, all in HBM."""
+    """n = 8*nb interior cells per axis with periodic boundaries, in one of
+    two layouts, all tables in HBM:
 
-    def __init__(self, nb, stream=None):
+    ghost=1 (the reference's): an 8-cell (one-brick) ghost zone -- a
+      (nb+2)^3 brick grid numbered by init_grid, its adjacency, the interior
+      brick list and the periodic ghost pairs a refresh copies before each
+      sweep;
+    ghost=0: an nb^3 brick grid whose adjacency wraps (init_grid with
+      periodic=True); every brick is interior and nothing needs refreshing.
+      A sweep reads the periodic image itself where the ghost layout reads
+      its copy, so both give the same values, bit for bit."""
+
+    def __init__(self, nb, stream=None, ghost: int = 1):
         self.nb = tuple(int(x) for x in nb)
-        self.padded = tuple(x + 2 for x in self.nb)
-        self.grid, self.info = init_grid(self.padded)
+        if ghost not in (0, 1):
+            raise ValueError("ghost must be 0 or 1")
+        self.ghost = int(ghost)
+        self.padded = tuple(x + 2 * self.ghost for x in self.nb)
+        self.grid, self.info = init_grid(self.padded, periodic=not self.ghost)
         movBrickInfo(self.info, stream)
-        n = C.c_uint32()
-        check(load().bk_interior_list(longs(self.nb), None, C.byref(n)))
-        self.interior = np.empty(n.value, dtype=np.uint32)
-        check(load().bk_interior_list(longs(self.nb), self.interior.ctypes.data_as(_lib.u32p),
-                                      C.byref(n)))
-        check(load().bk_ghost_pairs(longs(self.nb), None, C.byref(n)))
-        self.pairs = np.empty(2 * n.value, dtype=np.uint32)
-        check(load().bk_ghost_pairs(longs(self.nb), self.pairs.ctypes.data_as(_lib.u32p),
-                                    C.byref(n)))
+        if self.ghost:
+            n = C.c_uint32()
+            check(load().bk_interior_list(longs(self.nb), None, C.byref(n)))
+            self.interior = np.empty(n.value, dtype=np.uint32)
+            check(load().bk_interior_list(longs(self.nb), self.interior.ctypes.data_as(_lib.u32p),
+                                          C.byref(n)))
+            check(load().bk_ghost_pairs(longs(self.nb), None, C.byref(n)))
+            self.pairs = np.empty(2 * n.value, dtype=np.uint32)
+            check(load().bk_ghost_pairs(longs(self.nb), self.pairs.ctypes.data_as(_lib.u32p),
+                                        C.byref(n)))
+        else:
+            self.interior = np.ascontiguousarray(self.grid.reshape(-1), dtype=np.uint32)
+            self.pairs = np.empty(0, dtype=np.uint32)
         self.d_interior = DeviceBuffer.from_array(self.interior, stream)
-        self.d_pairs = DeviceBuffer.from_array(self.pairs, stream) if n.value else None
+        self.d_pairs = DeviceBuffer.from_array(self.pairs, stream) if self.pairs.size else None
         self.d_grid = DeviceBuffer.from_array(self.grid.reshape(-1), stream)
 
         self._plans: dict = {}
@@ -393,9 +408,9 @@ class PeriodicGrid:
 
     def fill_uniform(self, brick: Brick, seed: int, lo: float = -1.0, hi: float = 1.0,
                      stream=None) -> None:
-        check(load().bk_fill_uniform(brick.dtype, _vp(brick.ptr), C.c_size_t(brick.step),
-                                     longs(self.nb), C.c_uint64(seed), lo, hi,
-                                     _vp(_stream_of(stream))))
+        check(load().bk_fill_uniform_grid(brick.dtype, _vp(brick.ptr), C.c_size_t(brick.step),
+                                          longs(self.nb), self.ghost, C.c_uint64(seed), lo, hi,
+                                          _vp(_stream_of(stream))))
 
     def ghost_refresh(self, brick: Brick, stream=None) -> None:
         if not self.npairs:

@@ -88,14 +88,16 @@ __global__ __launch_bounds__(256) void ghost_batched(const BatchTable *tab_, uin
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t p = blockIdx.x * 4u + (uint32_t)w; p < total; p += gridDim.x * 4u) {
-    const uint32_t sidx = uniform(find_segment(tab->prefix, nsegs, p));
-    const auto &sg = tab->seg[sidx];
+  const uint32_t w = uniform(threadIdx.x >> 6);  // wave-uniform item, segment, pointers
+  const int lane = threadIdx.x & 63;
+  SegCursor cur;
+  for (uint32_t p = blockIdx.x * 4u + w; p < total; p += gridDim.x * 4u) {
+    cur.seek(tab->prefix, nsegs, p);
+    const auto &sg = tab->seg[cur.s];
     if (tick >= sg.ticks) continue;
     const bool odd = ((sg.parity + tick) & 1u) != 0;
     char *base = static_cast<char *>(odd ? sg.f1 : sg.f0);
-    const uint32_t q = p - tab->prefix[sidx];
+    const uint32_t q = p - cur.lo;
     const uint32_t dst = sg.items[2 * q], src = sg.items[2 * q + 1];
     const size_t step_bytes = (size_t)sg.step * (NV * 16 * 64 / 512);
     const uint4 *sp = reinterpret_cast<const uint4 *>(base + (size_t)src * step_bytes);
@@ -148,8 +150,8 @@ __device__ __forceinline__ uint64_t hash64(uint64_t seed, uint64_t n) {
 template <class T>
 __global__ __launch_bounds__(256) void fill_uniform(T *__restrict__ dat, uint64_t step,
                                                     uint32_t nb0, uint32_t nb1,
-                                                    uint32_t nb2, uint64_t seed,
-                                                    double lo, double hi) {
+                                                    uint32_t nb2, uint32_t ghost,
+                                                    uint64_t seed, double lo, double hi) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nbricks = (uint64_t)nb0 * nb1 * nb2;
   if (t >= nbricks * 512) return;
@@ -162,8 +164,8 @@ __global__ __launch_bounds__(256) void fill_uniform(T *__restrict__ dat, uint64_
   const uint64_t x = ti * 8 + i, y = tj * 8 + j, z = tk * 8 + k;
   const double u = (double)(hash64(seed, x + n0 * (y + n1 * z)) >> 11) * 0x1.0p-53;
   const double v = lo + (hi - lo) * u;
-  const uint64_t S0 = nb0 + 2, S1 = nb1 + 2;
-  const uint64_t b = (ti + 1) + S0 * ((tj + 1) + S1 * (tk + 1));
+  const uint64_t S0 = nb0 + 2 * ghost, S1 = nb1 + 2 * ghost;
+  const uint64_t b = (ti + ghost) + S0 * ((tj + ghost) + S1 * (tk + ghost));
   if constexpr (sizeof(T) == 8) {
     dat[b * step + e] = v;
   } else if constexpr (sizeof(T) == 4) {
@@ -384,7 +386,7 @@ hipError_t launch_zero(void *dst, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_fill_uniform(int dtype, void *dat, size_t step, const long nb[3],
+hipError_t launch_fill_uniform(int dtype, void *dat, size_t step, const long nb[3], int ghost,
                                uint64_t seed, double lo, double hi, hipStream_t s) {
   const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2] * 512;
   if (n == 0) return hipSuccess;
@@ -393,17 +395,17 @@ hipError_t launch_fill_uniform(int dtype, void *dat, size_t step, const long nb[
     case BK_F64:
       hipLaunchKernelGGL(fill_uniform<double>, dim3(nblk), dim3(256), 0, s,
                          static_cast<double *>(dat), (uint64_t)step, (uint32_t)nb[0],
-                         (uint32_t)nb[1], (uint32_t)nb[2], seed, lo, hi);
+                         (uint32_t)nb[1], (uint32_t)nb[2], (uint32_t)ghost, seed, lo, hi);
       break;
     case BK_F32:
       hipLaunchKernelGGL(fill_uniform<float>, dim3(nblk), dim3(256), 0, s,
                          static_cast<float *>(dat), (uint64_t)step, (uint32_t)nb[0],
-                         (uint32_t)nb[1], (uint32_t)nb[2], seed, lo, hi);
+                         (uint32_t)nb[1], (uint32_t)nb[2], (uint32_t)ghost, seed, lo, hi);
       break;
     case BK_BF16:
       hipLaunchKernelGGL(fill_uniform<bf16>, dim3(nblk), dim3(256), 0, s,
                          static_cast<bf16 *>(dat), (uint64_t)step, (uint32_t)nb[0],
-                         (uint32_t)nb[1], (uint32_t)nb[2], seed, lo, hi);
+                         (uint32_t)nb[1], (uint32_t)nb[2], (uint32_t)ghost, seed, lo, hi);
       break;
     default: return hipErrorInvalidValue;
   }

@@ -295,18 +295,20 @@ __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
 // Batched variant: one persistent launch serves every cube request of a
 // group table; work item = one quad-plan unit of one request.
 template <class T>
-__global__ __launch_bounds__(64) void cube_sweep_batched(const BatchTable *tab_, uint32_t tick) {
+__global__ __launch_bounds__(64, sizeof(T) == 8 ? 4 : 1) void cube_sweep_batched(const BatchTable *tab_,
+                                                                      uint32_t tick) {
   using A = typename acc_of<T>::type;
   __shared__ A plane[2][kQuadCells];
   __shared__ uint32_t tbl[16];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
+  SegCursor cur;
   for (uint32_t u = xcd_remap(blockIdx.x, gridDim.x); u < total; u += gridDim.x) {
-    const uint32_t sidx = uniform(find_segment(tab->prefix, nsegs, u));
-    const auto &sg = tab->seg[sidx];
+    cur.seek(tab->prefix, nsegs, u);
+    const auto &sg = tab->seg[cur.s];
     if (tick >= sg.ticks) continue;
-    const uint32_t ul = u - tab->prefix[sidx];
+    const uint32_t ul = u - cur.lo;
     const bool odd = ((sg.parity + tick) & 1u) != 0;
     const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
     T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);

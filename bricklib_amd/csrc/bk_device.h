@@ -89,4 +89,25 @@ __device__ __forceinline__ uint32_t find_segment(kconst_ptr<uint32_t> prefix, ui
   return lo;
 }
 
+// Segment cursor for a persistent batched loop in which each wave visits
+// its items in increasing order: the current and the next segment are
+// checked first, and the binary search runs only on a longer jump (or the
+// first item).  All members are wave-uniform.  A found segment is never
+// empty, so lo == hi marks the cursor as unset.
+struct SegCursor {
+  uint32_t s = 0, lo = 0, hi = 0;  // segment s holds items [lo, hi)
+
+  __device__ __forceinline__ void seek(kconst_ptr<uint32_t> prefix, uint32_t nsegs,
+                                       uint32_t item) {
+    if (item >= lo && item < hi) return;
+    if (hi > lo && item >= hi && s + 2 <= nsegs && item < prefix[s + 2]) {
+      s += 1;
+    } else {
+      s = find_segment(prefix, nsegs, item);
+    }
+    lo = prefix[s];
+    hi = prefix[s + 1];
+  }
+};
+
 }  // namespace bk

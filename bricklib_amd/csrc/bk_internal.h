@@ -93,8 +93,8 @@ hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,
 hipError_t launch_zero(void *dst, size_t bytes, hipStream_t s);
 hipError_t launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_fill_uniform(int dtype, void *dat, size_t step,
-                               const long nb[3], uint64_t seed, double lo,
-                               double hi, hipStream_t s);
+                               const long nb[3], int ghost, uint64_t seed,
+                               double lo, double hi, hipStream_t s);
 // Deterministic two-pass FP64 reductions; `scratch` holds kReduceBlocks*8
 // doubles.  Results land in scratch[0 .. nout).
 constexpr int kReduceBlocks = 1024;

@@ -257,10 +257,37 @@ int bk_storage_alloc(int dtype, long chunks, size_t step, void **d_dat) {
   return bk_alloc(bytes, d_dat);
 }
 
+// The default stream-ordered pool hands freed memory back to the driver at
+// every synchronisation (release threshold 0), so a loop that frees grids
+// and allocates new ones pays an unmap + map of every field each round.
+// Keep freed memory in the pool instead (per device, once).
+namespace {
+std::mutex g_pool_mu;
+bool g_pool_kept[64] = {};
+
+hipError_t keep_pool_memory() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipSuccess;
+  std::lock_guard<std::mutex> lock(g_pool_mu);
+  if (g_pool_kept[dev]) return hipSuccess;
+  hipMemPool_t pool = nullptr;
+  e = hipDeviceGetDefaultMemPool(&pool, dev);
+  if (e != hipSuccess) return e;
+  uint64_t keep = UINT64_MAX;
+  e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  if (e != hipSuccess) return e;
+  g_pool_kept[dev] = true;
+  return hipSuccess;
+}
+}  // namespace
+
 int bk_alloc_async(size_t bytes, void **d_ptr, bk_stream s) {
   if (!d_ptr) return fail(BK_ERR_ARG, "bk_alloc_async: null out pointer");
   *d_ptr = nullptr;
   if (bytes == 0) return BK_OK;
+  BK_HIP(keep_pool_memory());
   hipError_t e = hipMallocAsync(d_ptr, bytes, S(s));
   if (e == hipErrorOutOfMemory)
     return fail(BK_ERR_NOMEM, "bk_alloc_async: %zu bytes: out of memory", bytes);
@@ -558,15 +585,21 @@ int bk_graph_cache_clear(void) {
 // ---------------------------------------------------------------------------
 // seeded fill and FP64 reductions
 // ---------------------------------------------------------------------------
-int bk_fill_uniform(int dtype, void *d_dat, size_t step, const long nb[3], uint64_t seed,
-                    double lo, double hi, bk_stream s) {
+int bk_fill_uniform_grid(int dtype, void *d_dat, size_t step, const long nb[3], int ghost,
+                         uint64_t seed, double lo, double hi, bk_stream s) {
   if (!valid_dtype(dtype)) return fail(BK_ERR_ARG, "bk_fill_uniform: unknown dtype %d", dtype);
   if (!d_dat || !nb) return fail(BK_ERR_ARG, "bk_fill_uniform: null pointer");
   for (int a = 0; a < 3; ++a)
     if (nb[a] <= 0) return fail(BK_ERR_ARG, "bk_fill_uniform: nb must be positive");
+  if (ghost != 0 && ghost != 1) return fail(BK_ERR_ARG, "bk_fill_uniform: ghost must be 0 or 1");
   if (step < 512) return fail(BK_ERR_ARG, "bk_fill_uniform: step smaller than a brick");
-  BK_HIP(bk::launch_fill_uniform(dtype, d_dat, step, nb, seed, lo, hi, S(s)));
+  BK_HIP(bk::launch_fill_uniform(dtype, d_dat, step, nb, ghost, seed, lo, hi, S(s)));
   return BK_OK;
+}
+
+int bk_fill_uniform(int dtype, void *d_dat, size_t step, const long nb[3], uint64_t seed,
+                    double lo, double hi, bk_stream s) {
+  return bk_fill_uniform_grid(dtype, d_dat, step, nb, 1, seed, lo, hi, s);
 }
 
 int bk_field_stats(int dtype, const void *d_dat, size_t step, const uint32_t *d_list,

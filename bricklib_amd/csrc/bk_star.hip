@@ -409,31 +409,34 @@ __global__ __launch_bounds__(256) void star_sweep_batched(const BatchTable *tab_
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
-  const int w = threadIdx.x >> 6;
+  // the wave index through readfirstlane: everything derived from the item
+  // (segment, brick, pointers, coefficients) is then wave-uniform and lives
+  // in scalar registers, with scalar loads and uniform branches
+  const uint32_t w = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const uint32_t nwg = gridDim.x;
+  SegCursor cur;
   for (uint32_t base = xcd_remap(blockIdx.x, nwg) * 4u; base < total; base += nwg * 4u) {
-    const uint32_t item = base + (uint32_t)w;
+    const uint32_t item = base + w;
     bool active = item < total;
-    uint32_t sidx = 0, b = 0;
     if (active) {
-      sidx = uniform(find_segment(tab->prefix, nsegs, item));
-      active = tick < tab->seg[sidx].ticks;
+      cur.seek(tab->prefix, nsegs, item);
+      active = tick < tab->seg[cur.s].ticks;
     }
     StarBrick<T, R, COND> sb;
-    const T *in = nullptr;
-    T *out = nullptr;
+    uint32_t b = 0;
     if (active) {
-      const auto &sg = tab->seg[sidx];
+      const auto &sg = tab->seg[cur.s];
       const bool odd = ((sg.parity + tick) & 1u) != 0;
-      in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
-      out = static_cast<T *>(odd ? sg.f0 : sg.f1);
-      b = uniform(sg.items[item - tab->prefix[sidx]]);
+      const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
+      b = sg.items[item - cur.lo];
       sb.load(sg.adj, in, sg.step, b, tile[w], lane);
     }
     __syncthreads();
     if (active) {
-      const auto &sg = tab->seg[sidx];
+      const auto &sg = tab->seg[cur.s];
+      const bool odd = ((sg.parity + tick) & 1u) != 0;
+      T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);
       const kconst_ptr<A> cp = (kconst_ptr<A>)sg.coeff;
       A c[6 * R + 1];
 #pragma unroll

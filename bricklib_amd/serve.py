@@ -18,7 +18,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._lib import BK_F32, BK_F64
-from .batch import Batch, Request, ShapeCache
+from .batch import Arena, Batch, Request, ShapeCache
 from .bricks import Event, Stream
 from .stencils import NAMES, Stencil, u01
 
@@ -51,6 +51,7 @@ class ServeStats:
     completed: list = field(default_factory=list)
     elapsed: float = 0.0
     peak_bytes: int = 0
+    reserved_bytes: int = 0  # the arena's slabs at the end
     chunks: int = 0
 
     def latencies(self) -> np.ndarray:
@@ -66,6 +67,7 @@ class ServeStats:
                 "elapsed_s": self.elapsed,
                 "gstencil_s": work / self.elapsed / 1e9 if self.elapsed > 0 else 0.0,
                 "stencil_updates": work, "peak_hbm_bytes": self.peak_bytes,
+                "arena_reserved_bytes": self.reserved_bytes,
                 "chunks": self.chunks}
 
 
@@ -73,28 +75,43 @@ class Server:
     """Continuous-batching loop over one Batch."""
 
     def __init__(self, hbm_budget: float = 200e9, chunk: int = 8, max_active: int = 2048,
-                 use_graph: bool = True):
+                 use_graph: bool = True, ghost: int = 0, slab_bytes: int = 1 << 30):
         self.stream = Stream()
+        # request fields come from an arena used only on self.stream: a
+        # retired request's memory is reused at once, without a driver call
+        self.arena = Arena(slab_bytes)
         self.batch = Batch(max_active, stream=self.stream)
         self.shapes = ShapeCache()
         self.budget = float(hbm_budget)
         self.chunk = int(chunk)
         self.max_active = int(max_active)
         self.use_graph = use_graph
+        # field layout of admitted requests: 0 = wrapped adjacency (no ghost
+        # bricks, no refresh), 1 = one-brick ghost zone refreshed every tick
+        self.ghost = int(ghost)
 
     def admissible(self, r: Request, used: int, nactive: int) -> bool:
         return nactive < self.max_active and used + r.hbm_bytes <= self.budget
 
-    def serve(self, requests: list, realtime: bool = True, keep_results: bool = False,
+    def prewarm(self, requests: list) -> None:
+        """Build the shared per-shape tables of these requests' grid shapes
+        ahead of time (a shape catalogue known to the server)."""
+        for r in requests:
+            self.shapes.get(r.nb, self.ghost)
+        self.stream.sync()
+
+    def serve(self, requests: list, realtime: bool = True, keep_results=False,
               on_complete=None) -> ServeStats:
         """Serve `requests` (sorted by arrival).  realtime: arrivals follow
         the wall clock from the start; otherwise all are present at t=0
-        (a saturating, closed-loop run)."""
+        (a saturating, closed-loop run).  keep_results: True, or a set of
+        request ids whose fields stay allocated after completion."""
         pending = sorted(requests, key=lambda r: r.arrival)
         if not realtime:
             for r in pending:
                 r.arrival = 0.0
         for r in pending:
+            r.ghost = self.ghost
             if r.hbm_bytes > self.budget:
                 raise ValueError(f"request {r.rid} needs {r.hbm_bytes} B, above the budget")
         active: list = []
@@ -110,7 +127,7 @@ class Server:
                 r = pending[qi]
                 if not self.admissible(r, used, len(active)):
                     break
-                r.allocate(self.shapes, self.stream)
+                r.allocate(self.shapes, self.stream, self.arena)
                 r.admitted = now
                 used += r.hbm_bytes
                 active.append(r)
@@ -132,7 +149,9 @@ class Server:
                     r.finished = now
                     if on_complete is not None:
                         on_complete(r)
-                    if not keep_results:
+                    keep = keep_results if isinstance(keep_results, bool) else (
+                        r.rid in keep_results)
+                    if not keep:
                         r.release()
                     used -= r.hbm_bytes
                     st.completed.append(r)
@@ -140,6 +159,7 @@ class Server:
                     still.append(r)
             active = still
         st.elapsed = time.perf_counter() - t0
+        st.reserved_bytes = self.arena.stats()["reserved"]
         return st
 
 

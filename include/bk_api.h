@@ -75,6 +75,7 @@ typedef void *bk_stream; /* hipStream_t */
 typedef void *bk_event;  /* hipEvent_t */
 typedef struct bk_plan_s *bk_plan; /* execution plan (bk_plan_create) */
 typedef struct bk_batch_s *bk_batch; /* batched sweeps (bk_batch_create) */
+typedef struct bk_arena_s *bk_arena; /* field arena (bk_arena_create) */
 
 enum bk_plan_shape { BK_PLAN_COLUMN = 1, BK_PLAN_QUAD = 4 };
 
@@ -99,9 +100,25 @@ int bk_free(void *d_ptr);
 int bk_alloc(size_t bytes, void **d_ptr);
 /* Stream-ordered allocation and release (hipMallocAsync / hipFreeAsync):
  * no device-wide synchronisation, for loops that admit and retire grids
- * while other grids are being swept. */
+ * while other grids are being swept.  Memory freed this way stays in the
+ * device's default pool for reuse (release threshold set to the maximum on
+ * the first bk_alloc_async of each device). */
 int bk_alloc_async(size_t bytes, void **d_ptr, bk_stream s);
 int bk_free_async(void *d_ptr, bk_stream s);
+/* Field arena for grids that come and go (no counterpart in the reference,
+ * whose grids live for the whole run).  Blocks are carved best-fit out of
+ * slabs of at least slab_bytes (hipMalloc on the current device, kept until
+ * bk_arena_destroy; capacity bounds their total, 0 = no bound), 2 KiB
+ * aligned; a freed block merges with its free neighbours.  No driver call
+ * per block.  A block freed while queued kernels still use it may be handed
+ * out again at once: its next user must work on the same stream (or wait
+ * for it).  Thread-safe. */
+int bk_arena_create(size_t slab_bytes, size_t capacity, bk_arena *out);
+int bk_arena_destroy(bk_arena a);
+int bk_arena_alloc(bk_arena a, size_t bytes, void **d_ptr);
+int bk_arena_free(bk_arena a, void *d_ptr);
+int bk_arena_stats(bk_arena a, size_t *used, size_t *reserved, size_t *largest_free,
+                   uint32_t *live_blocks);
 /* Host<->device copies: movBrickStorage (brick-gpu.h:84-103) and
  * movBrickInfo (brick-gpu.h:42-55) are built on these.  Sizes are size_t
  * (the reference computes them in `unsigned`, brick-gpu.h:47,89). */
@@ -286,6 +303,14 @@ int bk_batch_run(bk_batch batch, uint32_t nticks, int use_graph, bk_stream s);
  * converted to dtype with round-to-nearest-even.  Ghost bricks untouched. */
 int bk_fill_uniform(int dtype, void *d_dat, size_t step, const long nb[3],
                     uint64_t seed, double lo, double hi, bk_stream s);
+/* The same values for a grid with `ghost` (0 or 1) ghost-brick layers:
+ * ghost = 1 is bk_fill_uniform; ghost = 0 fills an nb[0] x nb[1] x nb[2]
+ * brick grid numbered as bk_init_grid (the layout of a periodic grid whose
+ * adjacency wraps, bk_init_grid with periodic != 0, which needs no ghost
+ * refresh).  Interior cell (x, y, z) gets the same value in both layouts. */
+int bk_fill_uniform_grid(int dtype, void *d_dat, size_t step, const long nb[3],
+                         int ghost, uint64_t seed, double lo, double hi,
+                         bk_stream s);
 /* FP64 reductions over the bricks in d_list (brick_elems each):
  *   out[0] = sum x, out[1] = sum x^2, out[2] = max |x|
  * written to the host array h_out after a stream sync. */

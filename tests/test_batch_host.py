@@ -1,0 +1,37 @@
+"""Host side of the batched path (no GPU): the packed request records that
+bk_batch_set reads must match the C layout of bk_batch_req field by field."""
+import ctypes as C
+
+import numpy as np
+
+from bricklib_amd._lib import bk_batch_req, dblp
+from bricklib_amd.batch import REQ_DTYPE
+
+
+def test_record_dtype_matches_struct():
+    assert REQ_DTYPE.itemsize == C.sizeof(bk_batch_req)
+    for name, _ in bk_batch_req._fields_:
+        assert REQ_DTYPE.fields[name][1] == getattr(bk_batch_req, name).offset
+
+
+def test_packed_records_round_trip():
+    co = [np.arange(7, dtype=np.float64) + q for q in range(3)]
+    recs = []
+    for q in range(3):
+        r = bk_batch_req()
+        r.stencil, r.dtype = q, 1
+        r.d_adj, r.d_list, r.nlist = 0x1000 + q, 0x2000 + q, 10 + q
+        r.d_f0, r.d_f1, r.step = 0x3000 + q, 0x4000 + q, 512
+        r.h_coeff = co[q].ctypes.data_as(dblp)
+        r.theta, r.cond_mode = 0.25 * q, 1
+        recs.append(bytes(r))
+    arr = np.frombuffer(bytearray(b"".join(recs)), dtype=REQ_DTYPE)
+    arr["ticks"] = [5, 6, 7]
+    arr["parity"] = [1, 0, 1]
+    p = arr.ctypes.data_as(C.POINTER(bk_batch_req))
+    for q in range(3):
+        assert (p[q].stencil, p[q].dtype, p[q].nlist, p[q].step) == (q, 1, 10 + q, 512)
+        assert (p[q].d_adj, p[q].d_list, p[q].d_f0, p[q].d_f1) == (
+            0x1000 + q, 0x2000 + q, 0x3000 + q, 0x4000 + q)
+        assert p[q].h_coeff[6] == 6 + q and p[q].theta == 0.25 * q
+        assert (p[q].ticks, p[q].parity) == (5 + q, (q + 1) % 2)

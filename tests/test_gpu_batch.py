@@ -30,27 +30,47 @@ def make_mix():
     return reqs
 
 
-def alone(gpu, r: Request) -> np.ndarray:
-    """The same solve through the single-grid path."""
-    g = gpu.PeriodicGrid(r.nb)
+def alone(gpu, r: Request, ghost: int = 1) -> np.ndarray:
+    """The same solve through the single-grid path (ghost zone + refresh by
+    default): its interior bricks in grid order."""
+    g = gpu.PeriodicGrid(r.nb, ghost=ghost)
     a = gpu.Brick(g.info, g.field_storage(r.dtype), 0)
     b = gpu.Brick(g.info, g.field_storage(r.dtype), 0)
     g.fill_uniform(a, r.seed)
     res = gpu.sweeps(r.stencil, a, b, g, r.steps)
-    return res.storage.to_host(), g.interior
+    return res.storage.to_host()[g.interior]
 
 
-@pytest.mark.parametrize("chunk", [1, 3, 8])
-def test_mix_matches_alone(gpu, chunk):
-    srv = serve.Server(chunk=chunk)
+@pytest.mark.parametrize("chunk,ghost", [(1, 0), (3, 0), (8, 0), (3, 1), (8, 1)])
+def test_mix_matches_alone(gpu, chunk, ghost):
+    srv = serve.Server(chunk=chunk, ghost=ghost)
     reqs = make_mix()
     stats = srv.serve(reqs, realtime=False, keep_results=True)
     assert len(stats.completed) == len(reqs)
     for r in reqs:
-        ref, interior = alone(gpu, r)
-        got = r.result_host()
-        print(f"[case] rid={r.rid} {r.nb} kind={r.stencil.kind} dtype={r.dtype} steps={r.steps}")
-        np.testing.assert_array_equal(got[interior], ref[interior])
+        print(f"[case] rid={r.rid} {r.nb} kind={r.stencil.kind} dtype={r.dtype} "
+              f"steps={r.steps} ghost={ghost}", flush=True)
+        np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
+
+
+@pytest.mark.parametrize("name", ["7pt", "13pt", "25pt", "125pt", "cond"])
+def test_wrapped_single_grid_matches_ghost(gpu, name):
+    """The wrapped-adjacency layout (no ghost bricks, no refresh) gives the
+    ghost-zone layout's values bit for bit, with every schedule."""
+    nb = (3, 2, 5)
+    r = Request(rid=0, nb=nb, stencil=Stencil.make(name, seed=9), dtype=BK_F64, steps=4, seed=11)
+    ref = alone(gpu, r, ghost=1)
+    np.testing.assert_array_equal(alone(gpu, r, ghost=0), ref)
+    g = gpu.PeriodicGrid(nb, ghost=0)
+    assert g.npairs == 0 and g.nbricks == 30
+    for shape, kcol in (("column", 2), ("column", 8), ("quad", 2), ("quad", 8)):
+        if name == "125pt" and shape != "quad":
+            continue
+        a = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+        b = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+        g.fill_uniform(a, r.seed)
+        res = gpu.sweeps(r.stencil, a, b, g, r.steps, plan=g.plan(shape, kcol))
+        np.testing.assert_array_equal(res.storage.to_host()[g.interior], ref)
 
 
 def test_mix_fp64_matches_oracle(gpu):
@@ -63,7 +83,7 @@ def test_mix_fp64_matches_oracle(gpu):
         og.fill_uniform(oa, r.seed)
         ores = og.sweeps(r.stencil.kind, r.stencil.coeff, oa, ob, r.steps,
                          theta=r.stencil.theta, cond_mode=r.stencil.cond_mode)
-        np.testing.assert_array_equal(r.result_host()[og.interior], ores[og.interior])
+        np.testing.assert_array_equal(r.interior_host(), ores[og.interior])
 
 
 def test_tight_budget_queues_and_completes(gpu):
@@ -74,8 +94,7 @@ def test_tight_budget_queues_and_completes(gpu):
     assert len(stats.completed) == len(reqs)
     assert stats.peak_bytes <= budget
     for r in reqs[:4]:
-        ref, interior = alone(gpu, r)
-        np.testing.assert_array_equal(r.result_host()[interior], ref[interior])
+        np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
 
 
 def test_poisson_stream_small(gpu):
