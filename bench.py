@@ -66,15 +66,15 @@ def parse():
     return p.parse_args()
 
 
-def time_sweeps(bk, st, a, b, g, stream, steps, warmup, refresh=True):
+def time_sweeps(bk, st, a, b, g, stream, steps, warmup, refresh=True, **kw):
     """ms per sweep over `steps` ping-pong sweeps (events on `stream`)."""
     from bricklib_amd import Event
     if warmup:
-        bk.sweeps(st, a, b, g, warmup, refresh=refresh, stream=stream)
+        bk.sweeps(st, a, b, g, warmup, refresh=refresh, stream=stream, **kw)
     stream.sync()
     e0, e1 = Event(), Event()
     e0.record(stream)
-    bk.sweeps(st, a, b, g, steps, refresh=refresh, stream=stream)
+    bk.sweeps(st, a, b, g, steps, refresh=refresh, stream=stream, **kw)
     e1.record(stream)
     e1.sync()
     return e0.elapsed_ms(e1) / steps
@@ -319,6 +319,23 @@ def main():
                         "hbm_gbs": round(gb, 1), "hbm_frac": round(gb / HBM_PEAK_GBS, 4),
                         "tflops": round(tf, 2), "valu_frac": round(tf / peak_tf, 4)}
             print(f"[bench] {name}: {by[name]}", file=sys.stderr, flush=True)
+        # the same 7-point FP64 loop in two other forms: the refresh once per
+        # 8 sweeps (sweeps in between over every brick, weak/main.cpp:26-36),
+        # and the wrapped-adjacency layout (no ghost bricks, no refresh)
+        if not only or "7pt_f64_forms" in only:
+            cur.clear()
+            ms_deep = time_sweeps(bk, st7, a, b, g, stream, args.steps, 2, ghost_iters=8)
+            gw = bk.PeriodicGrid(nb, stream=stream, ghost=0)
+            wa = bk.Brick(gw.info, gw.field_storage(bk.BK_F64), 0)
+            wb = bk.Brick(gw.info, gw.field_storage(bk.BK_F64), 0)
+            gw.fill_uniform(wa, 1234, stream=stream)
+            ms_wrap = time_sweeps(bk, st7, wa, wb, gw, stream, args.steps, 2)
+            del wa, wb, gw
+            for name, ms_s in (("7pt_f64_refresh_every_8", ms_deep),
+                               ("7pt_f64_wrapped_no_ghost", ms_wrap)):
+                by[name] = {"gstencil_s": round(pts / (ms_s * 1e-3) / 1e9, 2),
+                            "ms_per_step": round(ms_s, 4)}
+                print(f"[bench] {name}: {by[name]}", file=sys.stderr, flush=True)
 
     prec = {}
     if not args.no_precision:

@@ -100,6 +100,8 @@ SIGNATURES = {
     "bk_interior_list": (C.c_int, [longp, u32p, u32p]),
     "bk_sweep": (C.c_int, [C.POINTER(bk_sweep_desc), vp]),
     "bk_ghost_refresh": (C.c_int, [C.c_int, vp, C.c_size_t, C.c_size_t, vp, C.c_uint32, vp]),
+    "bk_sweeps_deep": (C.c_int, [C.POINTER(bk_sweep_desc), vp, C.c_uint32, vp, vp, C.c_uint32,
+                                 C.c_int, C.c_int, C.c_int, vp]),
     "bk_sweeps": (C.c_int, [C.POINTER(bk_sweep_desc), vp, C.c_uint32, C.c_int, C.c_int, vp]),
     "bk_graph_cache_clear": (C.c_int, []),
     "bk_fill_uniform": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_uint64, C.c_double,

@@ -343,7 +343,7 @@ class PeriodicGrid:
       A sweep reads the periodic image itself where the ghost layout reads
       its copy, so both give the same values, bit for bit."""
 
-    def __init__(self, nb, stream=None, ghost: int = 1):
+    def __init__(self, nb, stream=None, ghost: int = 1, tile: int = 32):
         self.nb = tuple(int(x) for x in nb)
         if ghost not in (0, 1):
             raise ValueError("ghost must be 0 or 1")
@@ -366,16 +366,48 @@ class PeriodicGrid:
             self.pairs = np.empty(0, dtype=np.uint32)
         self.d_interior = DeviceBuffer.from_array(self.interior, stream)
         self.d_pairs = DeviceBuffer.from_array(self.pairs, stream) if self.pairs.size else None
+        # the order the sweeps visit the interior bricks in: grid order (k
+        # slowest) with tile=0, or the (i, j) plane cut into T x T brick
+        # tiles, each swept through all k before the next -- neighbours in k
+        # are then T*T list positions apart instead of a whole brick plane
+        # (T = 32 measured 4-6% faster for the 7-point star at 512^3).
+        # Every brick is computed the same way in any order.
+        self.tile = int(tile)
+        self.sweep_list = self.tiled_order(self.tile) if self.tile else self.interior
+        self.d_sweep_list = (DeviceBuffer.from_array(self.sweep_list, stream) if self.tile
+                             else self.d_interior)
         self.d_grid = DeviceBuffer.from_array(self.grid.reshape(-1), stream)
 
         self._plans: dict = {}
 
-    def plan(self, shape: str = "column", kcol: int = 16) -> Plan:
-        """A cached plan over the interior bricks."""
-        key = (shape, int(kcol))
+    def tiled_order(self, T: int) -> np.ndarray:
+        """Interior bricks tile by tile: T x T bricks in (i, j), all k."""
+        nb0, nb1, nb2 = self.nb
+        g = self.grid  # (k, j, i) padded numbering
+        o = self.ghost
+        out = []
+        for j0 in range(0, nb1, T):
+            for i0 in range(0, nb0, T):
+                blk = g[o:o + nb2, o + j0:o + min(j0 + T, nb1), o + i0:o + min(i0 + T, nb0)]
+                out.append(blk.reshape(-1))
+        return np.ascontiguousarray(np.concatenate(out), dtype=np.uint32)
+
+    def plan(self, shape: str = "column", kcol: int = 16, every: bool = False) -> Plan:
+        """A cached plan over the interior bricks in sweep order (every=True:
+        over every brick of the padded grid, ghost bricks included)."""
+        key = (shape, int(kcol), bool(every))
         if key not in self._plans:
-            self._plans[key] = Plan(self.info, self.interior, shape, kcol)
+            blist = np.arange(self.nbricks, dtype=np.uint32) if every else self.sweep_list
+            self._plans[key] = Plan(self.info, blist, shape, kcol)
         return self._plans[key]
+
+    @property
+    def d_every(self) -> DeviceBuffer:
+        """Every brick of the padded grid in grid order, in HBM (the sweep
+        list of brick_stencil(..., skip = 0), weak/main.cpp:26-36)."""
+        if getattr(self, "_d_every", None) is None:
+            self._d_every = DeviceBuffer.from_array(np.arange(self.nbricks, dtype=np.uint32))
+        return self._d_every
 
     def auto_plan(self, stencil_kind: int):
         """Default schedule per stencil, from measurements at 512^3 on
@@ -465,7 +497,7 @@ def _desc(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, coeff_keep: l
     d.stencil = st.kind
     d.dtype = inp.dtype
     d.d_adj = grid.info.dptr
-    d.d_list = (blist or grid.d_interior).ptr
+    d.d_list = (blist or grid.d_sweep_list).ptr
     d.nlist = grid.interior.size if nlist is None else nlist
     d.d_in = inp.ptr
     d.in_step = inp.step
@@ -491,15 +523,32 @@ def sweep(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, stream=None,
 
 
 def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
-           use_graph: bool = False, refresh: bool = True, stream=None, plan="auto") -> Brick:
+           use_graph: bool = False, refresh: bool = True, stream=None, plan="auto",
+           ghost_iters: int = 1) -> Brick:
     """``nsweeps`` ping-pong sweeps a->b->a..., each preceded by the periodic
-    ghost refresh of its input.  Returns the field holding the result."""
+    ghost refresh of its input -- or, with ghost_iters > 1, the refresh once
+    per ghost_iters sweeps and the sweeps in between over every brick of the
+    padded grid (bk_sweeps_deep; same result).  Returns the field holding
+    the result."""
     keep: list = []
     d = _desc(st, b, a, grid, keep, plan=plan)
     pairs = grid.d_pairs.ptr if (refresh and grid.d_pairs is not None) else None
     npairs = grid.npairs if refresh else 0
-    check(load().bk_sweeps(C.byref(d), _vp(pairs), C.c_uint32(npairs), int(nsweeps),
-                           int(bool(use_graph)), _vp(_stream_of(stream))))
+    if ghost_iters == 1:
+        check(load().bk_sweeps(C.byref(d), _vp(pairs), C.c_uint32(npairs), int(nsweeps),
+                               int(bool(use_graph)), _vp(_stream_of(stream))))
+    else:
+        # sweeps over every brick: column plans as for the interior; the
+        # cube per brick (quad plans cannot take the outer layer of a
+        # non-periodic grid, bk_plan_create rejects them)
+        ip = _resolve_plan(plan, st, grid)
+        ap = (grid.plan(ip.shape, ip.kcol, every=True)
+              if ip is not None and ip.shape != "quad" else None)
+        keep.append(ap)
+        check(load().bk_sweeps_deep(C.byref(d), _vp(grid.d_every.ptr), C.c_uint32(grid.nbricks),
+                                    _vp(ap.handle if ap is not None else None), _vp(pairs),
+                                    C.c_uint32(npairs), int(nsweeps), int(ghost_iters),
+                                    int(bool(use_graph)), _vp(_stream_of(stream))))
     return b if nsweeps % 2 == 1 else a
 
 

@@ -114,9 +114,155 @@ __global__ __launch_bounds__(256) void cube_sweep_lds(const CubeArgs<T> a) {
 // kernel, so results are identical.
 constexpr int kQuadPW = 20, kQuadCells = kQuadPW * kQuadPW;
 
+// One input plane P of the quad unit: accumulate its taps into the five
+// rolling slots (slot s holds output plane P - 2 + s, tap dk = 2 - s).
+// Taps go row by row of the LDS plane (r), then slot by slot; per output
+// point that is (dk, dj, di) lexicographic, the first tap a multiply.
+//
+// Edge planes (some slots outside the unit's output range): coefficients
+// are read group by group (row r, slot s: up to ten values) through a
+// constant-address pointer, so they arrive by scalar loads into SGPRs; the
+// empty asm at each group hides the pointer's loop invariance so the
+// compiler does not try to keep all 125 (250 SGPRs) live across planes.
+template <class A>
+__device__ __forceinline__ void quad_plane_edge(const A *pl, int lj, int li, kconst_ptr<A> cp,
+                                                int P, int nout, A (&acc)[5][2][2]) {
+  constexpr int PW = kQuadPW;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    A x[6];
+    // 16-byte LDS reads of the lane's two adjacent columns: 8 lanes of a
+    // row cover 128 contiguous bytes, so a read is free of bank conflicts
+#pragma unroll
+    for (int c = 0; c < 6; c += 2) {
+      typedef A A2 __attribute__((ext_vector_type(2)));
+      const A2 v = *reinterpret_cast<const A2 *>(&pl[(2 * lj + r) * PW + 2 * li + c]);
+      x[c] = v.x;
+      x[c + 1] = v.y;
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+      asm volatile("" : "+s"(cp));
+      const int O = P - 2 + s;
+      if (O < 0 || O >= nout) continue;
+      const int dk = 2 - s;
+#pragma unroll
+      for (int pj = 0; pj < 2; ++pj) {
+        const int dj = r - pj - 2;
+        if (dj < -2 || dj > 2) continue;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+#pragma unroll
+          for (int pi = 0; pi < 2; ++pi) {
+            const int di = c - pi - 2;
+            if (di < -2 || di > 2) continue;
+            const A cf = cp[(dk + 2) * 25 + (dj + 2) * 5 + (di + 2)];
+            if (s == 4 && dj == -2 && di == -2) acc[s][pj][pi] = cf * x[c];
+            else acc[s][pj][pi] = fmadd(cf, x[c], acc[s][pj][pi]);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Coefficient group (r, s) of a plane: the rows dj = r-3 .. r-2 that exist
+// (one row for r = 0 and r = 5) of tap plane dk = 2 - s -- up to ten
+// consecutive values of the (dk, dj, di)-ordered table.
+__device__ constexpr int quad_group_first_row(int r) { return r > 0 ? r - 1 : 0; }
+__device__ constexpr int quad_group_count(int r) { return (r == 0 || r == 5) ? 5 : 10; }
+
+// Steady-state planes (all five slots inside the unit): the same taps in
+// the same per-point order, with the scalar loads of group g+1's
+// coefficients and the LDS reads of row r+1 issued before group g's FMAs.
+// Scalar loads and LDS reads share one wait counter and scalar loads
+// complete out of order, so each group starts with one full wait.  The
+// empty asm statements pin the schedule: a "use" of the group's registers
+// puts the wait before the next group's loads; the launder of the pointer
+// (and of the LDS row offset) takes an accumulator of the previous group
+// as input, so the next loads cannot be hoisted above that group's FMAs
+// (which would keep every group's coefficients live at once).
+template <int G, class A>
+__device__ __forceinline__ void quad_steady_group(const A *pl, int xo, kconst_ptr<A> &cp,
+                                                  A (&cf)[2][10], A (&x)[2][6],
+                                                  A (&acc)[5][2][2]) {
+  constexpr int PW = kQuadPW;
+  typedef A A2 __attribute__((ext_vector_type(2)));
+  constexpr int r = G / 5, s = G % 5, cur = G & 1;
+  // an accumulator the previous group (rp, sp) wrote
+  constexpr int rp = (G + 29) / 5 % 6, sp = (G + 29) % 5, pjp = rp <= 4 ? 0 : 1;
+  // wait here for this group's coefficients (and, at a row start, the row)
+#pragma unroll
+  for (int q = 0; q < quad_group_count(r); ++q) asm volatile("" ::"s"(cf[cur][q]));
+  if constexpr (s == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) asm volatile("" ::"v"(x[r & 1][c]));
+  }
+  // issue the next group's scalar loads and, at a row start, the next row
+  if constexpr (G + 1 < 30) {
+    constexpr int rn = (G + 1) / 5, sn = (G + 1) % 5;
+    if constexpr (G > 0) asm volatile("" : "+s"(cp) : "v"(acc[sp][pjp][0]));
+    else asm volatile("" : "+s"(cp));
+    constexpr int off = (4 - sn) * 25 + quad_group_first_row(rn) * 5;
+#pragma unroll
+    for (int q = 0; q < quad_group_count(rn); ++q) cf[cur ^ 1][q] = cp[off + q];
+  }
+  if constexpr (s == 0 && r + 1 < 6) {
+    int ro = xo + (r + 1) * PW;
+    if constexpr (G > 0) asm volatile("" : "+v"(ro) : "v"(acc[sp][pjp][0]));
+    else asm volatile("" : "+v"(ro));
+#pragma unroll
+    for (int c = 0; c < 6; c += 2) {
+      const A2 v = *reinterpret_cast<const A2 *>(&pl[ro + c]);
+      x[(r + 1) & 1][c] = v.x;
+      x[(r + 1) & 1][c + 1] = v.y;
+    }
+  }
+  // this group's FMAs
+  constexpr int row0 = quad_group_first_row(r);
+#pragma unroll
+  for (int pj = 0; pj < 2; ++pj) {
+    const int dj = r - pj - 2;
+    if (dj < -2 || dj > 2) continue;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+      for (int pi = 0; pi < 2; ++pi) {
+        const int di = c - pi - 2;
+        if (di < -2 || di > 2) continue;
+        const A k = cf[cur][(dj + 2 - row0) * 5 + (di + 2)];
+        if (s == 4 && dj == -2 && di == -2) acc[s][pj][pi] = k * x[r & 1][c];
+        else acc[s][pj][pi] = fmadd(k, x[r & 1][c], acc[s][pj][pi]);
+      }
+    }
+  }
+  if constexpr (G + 1 < 30) quad_steady_group<G + 1, A>(pl, xo, cp, cf, x, acc);
+}
+
+template <class A>
+__device__ __forceinline__ void quad_plane_steady(const A *pl, int lj, int li,
+                                                  kconst_ptr<A> cp, A (&acc)[5][2][2]) {
+  typedef A A2 __attribute__((ext_vector_type(2)));
+  A cf[2][10];
+  A x[2][6];
+  const int xo = (2 * lj) * kQuadPW + 2 * li;
+  asm volatile("" : "+s"(cp));
+#pragma unroll
+  for (int q = 0; q < quad_group_count(0); ++q) cf[0][q] = cp[4 * 25 + q];
+#pragma unroll
+  for (int c = 0; c < 6; c += 2) {
+    const A2 v = *reinterpret_cast<const A2 *>(&pl[xo + c]);
+    x[0][c] = v.x;
+    x[0][c + 1] = v.y;
+  }
+  quad_steady_group<0, A>(pl, xo, cp, cf, x, acc);
+}
+
 // One plan unit of the quad schedule: members m0.. over L levels, store
 // masks mk[0..L), coefficients through the constant-address pointer cp.
-// plane: two LDS plane buffers, tbl: 16-entry LDS brick table; one wave.
+// plane: two LDS plane buffers, tbl: 32-entry LDS brick table (the current
+// input level's 4x4 block of columns, then the unit's level-0 block); one
+// wave.
 template <class T>
 __device__ __forceinline__ void cube_quad_unit(
     const uint32_t *__restrict__ adj, const T *__restrict__ in, T *__restrict__ out, size_t st,
@@ -130,13 +276,23 @@ __device__ __forceinline__ void cube_quad_unit(
   const uint32_t m2 = adj[(size_t)m0 * 27 + 16];
   const uint32_t m3 = adj[(size_t)m1 * 27 + 16];
 
-  // input-level brick table for level -1: the k-1 neighbours of the 4x4
-  // block of columns around the quad (each member's in-plane neighbours)
-  if (lane < 36) {
-    const int mm = lane / 9, q = lane - mm * 9, jd = q / 3, id = q - jd * 3;
-    const uint32_t m = mm == 0 ? m0 : (mm == 1 ? m1 : (mm == 2 ? m2 : m3));
-    const uint32_t n0 = adj[(size_t)m * 27 + 9 + 3 * jd + id];
-    tbl[((mm >> 1) + jd) * 4 + (mm & 1) + id] = adj[(size_t)n0 * 27 + 4];
+  // brick tables of the 4x4 block of columns around the quad: level 0 in
+  // tbl[16..32), its k-1 neighbours (the input level of the first two
+  // planes) in tbl[0..16).  Entry (r, c) comes from one member's in-plane
+  // neighbours: m0 for r, c < 3, m1 for column 3, m2 for row 3, m3 for the
+  // corner -- one writer per entry.  Level 0 is taken from the in-plane
+  // neighbours directly, not by walking up from level -1 (where level -1
+  // lies outside the grid its adjacency entries stand for no brick).  The
+  // plan builder has checked that every stored member's 27 neighbours are
+  // what these tables give.
+  if (lane < 16) {
+    const int r = lane >> 2, c = lane & 3;
+    const uint32_t m = r < 3 ? (c < 3 ? m0 : m1) : (c < 3 ? m2 : m3);
+    const int jd = r < 3 ? r : 2, id = c < 3 ? c : 2;
+    const int jj = (r == 3) ? 2 : jd, ii = (c == 3) ? 2 : id;
+    const uint32_t n0 = adj[(size_t)m * 27 + 9 + 3 * jj + ii];
+    tbl[lane] = adj[(size_t)n0 * 27 + 4];
+    tbl[16 + lane] = n0;
   }
   __syncthreads();
 
@@ -154,7 +310,7 @@ __device__ __forceinline__ void cube_quad_unit(
     base[t] = (size_t)tbl[tix[t]] * st;
   }
   uint32_t tn = 0;  // next level's table entry (lanes 0..15)
-  if (lane < 16) tn = adj[(size_t)tbl[lane] * 27 + 22];
+  if (lane < 16) tn = tbl[16 + lane];
 
   const int mm_l = (lj >> 2) * 2 + (li >> 2);
   const int jo = (2 * lj) & 7, io = (2 * li) & 7;
@@ -198,48 +354,11 @@ __device__ __forceinline__ void cube_quad_unit(
     }
 
     const A *pl = plane[buf];
-    // Coefficients are read group by group (row r, slot s: ten values)
-    // through a constant-address pointer, so they arrive by scalar loads
-    // into SGPRs; the empty asm at each group hides the pointer's loop
-    // invariance so the compiler does not try to keep all 125 (250 SGPRs)
-    // live across the plane loop.
-    kconst_ptr<A> cp = cp0;
-#pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      A x[6];
-      // 16-byte LDS reads of the lane's two adjacent columns: 8 lanes of a
-      // row cover 128 contiguous bytes, so a read is free of bank conflicts
-#pragma unroll
-      for (int c = 0; c < 6; c += 2) {
-        typedef A A2 __attribute__((ext_vector_type(2)));
-        const A2 v = *reinterpret_cast<const A2 *>(&pl[(2 * lj + r) * PW + 2 * li + c]);
-        x[c] = v.x;
-        x[c + 1] = v.y;
-      }
-#pragma unroll
-      for (int s = 0; s < 5; ++s) {
-        asm volatile("" : "+s"(cp));
-        const int O = P - 2 + s;
-        if (O < 0 || O >= nout) continue;
-        const int dk = 2 - s;
-#pragma unroll
-        for (int pj = 0; pj < 2; ++pj) {
-          const int dj = r - pj - 2;
-          if (dj < -2 || dj > 2) continue;
-#pragma unroll
-          for (int c = 0; c < 6; ++c) {
-#pragma unroll
-            for (int pi = 0; pi < 2; ++pi) {
-              const int di = c - pi - 2;
-              if (di < -2 || di > 2) continue;
-              const A cf = cp[(dk + 2) * 25 + (dj + 2) * 5 + (di + 2)];
-              if (s == 4 && dj == -2 && di == -2) acc[s][pj][pi] = cf * x[c];
-              else acc[s][pj][pi] = fmadd(cf, x[c], acc[s][pj][pi]);
-            }
-          }
-        }
-      }
-    }
+    // FP64 keeps the edge form throughout: the pipelined form's extra live
+    // registers (two rows, two coefficient groups) cost it a wave per SIMD
+    // (155 VGPRs) for no gain; FP32 gains about 2% from it
+    if (sizeof(A) == 4 && P >= 2 && P + 2 < nout) quad_plane_steady<A>(pl, lj, li, cp0, acc);
+    else quad_plane_edge<A>(pl, lj, li, cp0, P, nout, acc);
 
     const int O = P - 2;  // slot 0 is complete
     if (O >= 0 && O < nout) {
@@ -279,7 +398,7 @@ template <class T>
 __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
   using A = typename acc_of<T>::type;
   __shared__ A plane[2][kQuadCells];
-  __shared__ uint32_t tbl[16];
+  __shared__ uint32_t tbl[32];
   const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
   if (u >= a.nunits) return;
   // The argument struct is the kernel's only parameter and sits at offset 0
@@ -299,7 +418,7 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 4 : 1) void cube_sweep_batched
                                                                       uint32_t tick) {
   using A = typename acc_of<T>::type;
   __shared__ A plane[2][kQuadCells];
-  __shared__ uint32_t tbl[16];
+  __shared__ uint32_t tbl[32];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];

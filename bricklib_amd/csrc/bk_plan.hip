@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -93,11 +94,56 @@ void chain_cells(const uint32_t *adj, const std::vector<uint32_t> &cell_anchor,
       const uint32_t up = adj[(size_t)cell_anchor[cur] * 27 + 22];
       auto it = by_anchor.find(up);
       if (it == by_anchor.end() || used[it->second]) break;
+      // the kernels hand a level's top planes to the next level as its k-1
+      // face: link only where the adjacency agrees both ways (it does not
+      // on the outer layer of a non-periodic grid, whose out-of-range
+      // entries stand for no brick)
+      if (adj[(size_t)up * 27 + 4] != cell_anchor[cur]) break;
       cur = it->second;
       if (u.masks.empty()) u.anchor = cell_anchor[cur];
     }
     if (!u.masks.empty()) units.push_back(u);
   }
+}
+
+// The quad kernel derives every input brick of a unit from its level-0
+// anchor: members m1 = adj[m0][i+1], m2 = adj[m0][j+1], m3 = adj[m1][j+1];
+// a 4x4 in-plane table of columns (entry (r, c) from m0 for r, c < 3, m1
+// for column 3, m2 for row 3, m3 for the corner), level 0 from the members'
+// in-plane neighbours, level -1 through adj[.][k-1], higher levels walked
+// up through adj[.][k+1].  True when every stored member's 27 neighbours at
+// every level are what those tables give -- always on a periodic grid and
+// on the interior of a padded one, not on the outer layer of a
+// non-periodic grid.
+bool quad_unit_consistent(const uint32_t *adj, const Unit &u) {
+  auto A = [adj](uint32_t b, int s) { return adj[(size_t)b * 27 + s]; };
+  const uint32_t m0 = u.anchor, m1 = A(m0, 14), m2 = A(m0, 16), m3 = A(m1, 16);
+  const int L = (int)u.masks.size();
+  std::vector<std::array<uint32_t, 16>> T((size_t)L + 2);  // T[l + 1]: level l
+  for (int e = 0; e < 16; ++e) {
+    const int r = e >> 2, c = e & 3;
+    const uint32_t m = r < 3 ? (c < 3 ? m0 : m1) : (c < 3 ? m2 : m3);
+    const uint32_t n0 = A(m, 9 + 3 * (r < 3 ? r : 2) + (c < 3 ? c : 2));
+    T[1][e] = n0;
+    T[0][e] = A(n0, 4);
+  }
+  for (int l = 1; l <= L; ++l)
+    for (int e = 0; e < 16; ++e) T[(size_t)l + 1][e] = A(T[l][e], 22);
+  uint32_t om[4] = {m0, m1, m2, m3};
+  for (int l = 0; l < L; ++l) {
+    if (l > 0)
+      for (auto &b : om) b = A(b, 22);
+    for (int mm = 0; mm < 4; ++mm) {
+      if (!((u.masks[l] >> mm) & 1u)) continue;
+      for (int kd = 0; kd < 3; ++kd)
+        for (int jd = 0; jd < 3; ++jd)
+          for (int id = 0; id < 3; ++id)
+            if (A(om[mm], kd * 9 + jd * 3 + id) !=
+                T[(size_t)l + kd][((mm >> 1) + jd) * 4 + (mm & 1) + id])
+              return false;
+    }
+  }
+  return true;
 }
 
 }  // namespace
@@ -159,6 +205,13 @@ int plan_build(const uint32_t *h_adj, uint32_t nbricks, const uint32_t *h_list, 
   }
   std::vector<Unit> units;
   chain_cells(h_adj, cell_anchor, cell_mask, kcol, units);
+  if (shape == BK_PLAN_QUAD)
+    for (const Unit &u : units)
+      if (!quad_unit_consistent(h_adj, u))
+        return bk::plan_fail(BK_ERR_UNSUPPORTED,
+                             "bk_plan: a quad of the list has neighbours the adjacency does not "
+                             "place consistently (the outer layer of a non-periodic grid); use "
+                             "a column plan or no plan for such lists");
   // emit units in the list order of their anchors (for a grid-ordered list:
   // k-segment by k-segment, so units that run together are neighbours)
   {

@@ -146,8 +146,14 @@ hipError_t launch_sweep(const bk::SweepLaunch &L, const bk_plan_s *plan, hipStre
 }
 
 // Enqueue the ping-pong sequence of bk_sweeps on stream s.
+// The ping-pong sequence: before sweep t the ghost bricks of its input are
+// refreshed when t % ghost_iters == 0; sweep t covers the desc's brick list
+// (or plan) when it is the last of its refresh group or of the sequence,
+// and the `all` list (or plan) -- every brick of the padded grid, ghosts
+// included -- otherwise.  ghost_iters == 1 is the refresh-every-sweep loop.
 hipError_t enqueue_sweeps(const bk_sweep_desc *d, const uint32_t *pairs, uint32_t npairs,
-                          int nsweeps, hipStream_t s) {
+                          int nsweeps, int ghost_iters, const uint32_t *all, uint32_t nall,
+                          bk_plan all_plan, hipStream_t s) {
   bk::SweepLaunch L;
   to_launch(d, L);
   const size_t es = (size_t)dt_size(d->dtype);
@@ -157,15 +163,18 @@ hipError_t enqueue_sweeps(const bk_sweep_desc *d, const uint32_t *pairs, uint32_
     void *dst = fwd ? d->d_out : const_cast<void *>(d->d_in);
     const size_t sst = fwd ? d->in_step : d->out_step;
     const size_t dstep = fwd ? d->out_step : d->in_step;
-    if (npairs) {
+    if (npairs && t % ghost_iters == 0) {
       hipError_t e = bk::launch_ghost_refresh(src, sst * es, 512 * es, pairs, npairs, s);
       if (e != hipSuccess) return e;
     }
+    const bool last = t % ghost_iters == ghost_iters - 1 || t == nsweeps - 1;
     L.in = src;
     L.in_step = sst;
     L.out = dst;
     L.out_step = dstep;
-    hipError_t e = launch_sweep(L, d->plan, s);
+    L.list = last ? d->d_list : all;
+    L.nlist = last ? d->nlist : nall;
+    hipError_t e = launch_sweep(L, last ? d->plan : all_plan, s);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -506,15 +515,14 @@ int bk_ghost_refresh(int dtype, void *d_dat, size_t step, size_t brick_elems,
   return BK_OK;
 }
 
-int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, int nsweeps,
-              int use_graph, bk_stream s) {
-  int rc = check_desc(d);
-  if (rc != BK_OK) return rc;
-  if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps: negative sweep count");
-  if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps: null pair list");
+namespace {
+
+int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, int nsweeps,
+                int ghost_iters, const uint32_t *d_all, uint32_t nall, bk_plan all_plan,
+                int use_graph, hipStream_t s) {
   if (nsweeps == 0 || desc_empty(d)) return BK_OK;
   if (!use_graph) {
-    BK_HIP(enqueue_sweeps(d, d_pairs, npairs, nsweeps, S(s)));
+    BK_HIP(enqueue_sweeps(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan, s));
     return BK_OK;
   }
   // key: every argument that shapes the captured launches
@@ -539,6 +547,10 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, 
   put(&d_pairs, sizeof(d_pairs));
   put(&npairs, sizeof(npairs));
   put(&nsweeps, sizeof(nsweeps));
+  put(&ghost_iters, sizeof(ghost_iters));
+  put(&d_all, sizeof(d_all));
+  put(&nall, sizeof(nall));
+  put(&all_plan, sizeof(all_plan));
   auto &gc = graphs();
   std::lock_guard<std::mutex> lock(gc.mu);
   auto it = gc.exec.find(key);
@@ -556,7 +568,8 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, 
       cs = cit->second;
     }
     BK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_sweeps(d, d_pairs, npairs, nsweeps, cs);
+    hipError_t e =
+        enqueue_sweeps(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan, cs);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e != hipSuccess) {
@@ -570,8 +583,43 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, 
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
     it = gc.exec.emplace(key, ge).first;
   }
-  BK_HIP(hipGraphLaunch(it->second, S(s)));
+  BK_HIP(hipGraphLaunch(it->second, s));
   return BK_OK;
+}
+
+}  // namespace
+
+int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, int nsweeps,
+              int use_graph, bk_stream s) {
+  int rc = check_desc(d);
+  if (rc != BK_OK) return rc;
+  if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps: negative sweep count");
+  if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps: null pair list");
+  return sweeps_impl(d, d_pairs, npairs, nsweeps, 1, d->d_list, d->nlist, d->plan, use_graph,
+                     S(s));
+}
+
+int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
+                   bk_plan all_plan, const uint32_t *d_pairs, uint32_t npairs, int nsweeps,
+                   int ghost_iters, int use_graph, bk_stream s) {
+  int rc = check_desc(d);
+  if (rc != BK_OK) return rc;
+  if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps_deep: negative sweep count");
+  if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps_deep: null pair list");
+  const int R = bk_stencil_radius(d->stencil);
+  if (ghost_iters < 1 || R * ghost_iters > 8)
+    return fail(BK_ERR_ARG,
+                "bk_sweeps_deep: ghost_iters must be in [1, 8 / radius] (%d sweeps of radius %d "
+                "exceed the 8-cell ghost zone)",
+                ghost_iters, R);
+  bk_sweep_desc all = *d;
+  all.d_list = d_all;
+  all.nlist = nall;
+  all.plan = all_plan;
+  rc = check_desc(&all);
+  if (rc != BK_OK) return rc;
+  return sweeps_impl(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan,
+                     use_graph, S(s));
 }
 
 int bk_graph_cache_clear(void) {

@@ -172,7 +172,12 @@ int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n);
  * up to kcol levels of 2x2 blocks of columns (BK_PLAN_QUAD; members
  * anchor, adj[anchor][14], adj[anchor][16], adj[adj[anchor][14]][16]).
  * Every listed brick is written exactly once by a sweep that uses the plan;
- * bricks a unit computes but that are not listed are not written.  The
+ * bricks a unit computes but that are not listed are not written.  Columns
+ * link b to adj[b][k+1] only where adj[adj[b][k+1]][k-1] == b.  A quad plan
+ * requires every listed brick's 27 neighbours to be where the quad's 4x4
+ * column table puts them (true on periodic grids and on the interior of a
+ * padded grid, not on the outer layer of a non-periodic one): otherwise
+ * BK_ERR_UNSUPPORTED.  The
  * reference has no counterpart: its sweep is a loop over grid[k][j][i]
  * (stencils/3axis.cpp:51-66, weak/main.hip.cpp:274-279) and the plan is
  * how that loop is scheduled on the GPU.  Allocates device memory.
@@ -247,6 +252,21 @@ int bk_ghost_refresh(int dtype, void *d_dat, size_t step, size_t brick_elems,
  * and replayed. */
 int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs,
               int nsweeps, int use_graph, bk_stream s);
+/* The communication-avoiding form of the same loop (weak/main.cpp:26-36,
+ * :203-211, where brick_stencil(..., skip = 0) sweeps every brick of the
+ * padded grid and the last sweep per exchange uses skip = 1): the ghost
+ * bricks are refreshed once per ghost_iters sweeps; the sweeps in between
+ * cover every brick in d_all (or all_plan), ghost bricks included, and the
+ * last sweep of each group and of the sequence covers d->d_list (or
+ * d->plan).  Stale ghost values move inwards by the stencil radius per
+ * sweep, so the interior is exact -- bit for bit the bk_sweeps result --
+ * when radius * ghost_iters <= 8 (the one-brick ghost zone); larger values
+ * are rejected.  ghost_iters = 1 is bk_sweeps.  all_plan: a column plan
+ * over d_all, or NULL (one brick per wave); bk_plan_create rejects quad
+ * plans over the outer layer of a non-periodic grid. */
+int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
+                   bk_plan all_plan, const uint32_t *d_pairs, uint32_t npairs,
+                   int nsweeps, int ghost_iters, int use_graph, bk_stream s);
 /* Drop every cached graph (frees their device resources). */
 int bk_graph_cache_clear(void);
 

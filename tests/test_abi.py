@@ -102,6 +102,17 @@ def test_argument_errors_without_gpu(bk):
     assert lib.bk_storage_alloc(0, 1 << 62, 1 << 20, C.byref(p)) == BK_ERR_ARG
     assert lib.bk_init_grid(longs([0, 1, 1]), 0, None, None) == BK_ERR_ARG
     assert lib.bk_sweeps(C.byref(d), None, 0, -1, 0, None) == BK_ERR_ARG
+    d.in_step = 512
+    every = 12288
+    assert lib.bk_sweeps_deep(C.byref(d), every, 8, None, None, 0, 4, 0, 0, None) == BK_ERR_ARG
+    assert lib.bk_sweeps_deep(C.byref(d), every, 8, None, None, 0, 4, 9, 0, None) == BK_ERR_ARG
+    assert b"ghost zone" in lib.bk_last_error()
+    d.stencil = 2  # 25-point star, radius 4: at most 2 sweeps per refresh
+    coeff25 = np.ones(25)
+    d.h_coeff = coeff25.ctypes.data_as(C.POINTER(C.c_double))
+    assert lib.bk_sweeps_deep(C.byref(d), every, 8, None, None, 0, 4, 3, 0, None) == BK_ERR_ARG
+    assert lib.bk_sweeps_deep(C.byref(d), every, 8, None, None, 0, -1, 2, 0, None) == BK_ERR_ARG
+    assert lib.bk_sweeps_deep(C.byref(d), None, 8, None, None, 0, 4, 2, 0, None) == BK_ERR_ARG
 
 
 def test_python_mirror_raises(bk):

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from bricklib_amd import stencils as S
+from bricklib_amd import BK_F64, stencils as S
 
 pytestmark = pytest.mark.gpu
 
@@ -280,3 +280,40 @@ def test_stream_copy(gpu):
         _lib.check(lib.bk_stream_copy(C.c_void_p(dd.ptr), C.c_void_p(ds.ptr), nbytes, None))
         np.testing.assert_array_equal(dd.download(np.uint8), src)
     assert lib.bk_stream_copy(C.c_void_p(dd.ptr), C.c_void_p(dd.ptr), 16, None) == 1
+
+
+DEEP = [("7pt", 8), ("13pt", 4), ("25pt", 2), ("125pt", 4), ("cond", 8)]
+
+
+@pytest.mark.parametrize("name,gi", DEEP)
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_deep_ghost_matches_refresh_every_sweep(gpu, name, gi, use_graph):
+    """bk_sweeps_deep (refresh once per ghost_iters sweeps, sweeps over every
+    brick in between) gives the refresh-every-sweep result bit for bit, for
+    the largest ghost_iters the 8-cell ghost zone allows and a sweep count
+    that is not a multiple of it."""
+    from bricklib_amd import BK_F32
+    nb = (3, 2, 4)
+    st = S.Stencil.make(name, seed=21)
+    n = 2 * gi + 3
+    for dt in ((BK_F64, BK_F32) if name in ("7pt", "125pt") else (BK_F64,)):
+        g = gpu.PeriodicGrid(nb)
+        outs = []
+        for k in (1, gi):
+            a = gpu.Brick(g.info, g.field_storage(dt), 0)
+            b = gpu.Brick(g.info, g.field_storage(dt), 0)
+            g.fill_uniform(a, 5)
+            res = gpu.sweeps(st, a, b, g, n, use_graph=use_graph, ghost_iters=k)
+            outs.append(res.storage.to_host()[g.interior])
+        print(f"[case] deep {name} gi={gi} dt={dt} graph={use_graph}", flush=True)
+        np.testing.assert_array_equal(outs[1], outs[0])
+
+
+def test_deep_ghost_rejects_too_many_sweeps(gpu):
+    from bricklib_amd import BrickError
+    g = gpu.PeriodicGrid((2, 2, 2))
+    st = S.Stencil.make("25pt", seed=2)
+    a = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    b = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    with pytest.raises(BrickError):
+        gpu.sweeps(st, a, b, g, 4, ghost_iters=3)

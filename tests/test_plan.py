@@ -78,3 +78,52 @@ def test_plan_rejects_bad_input(bk):
         bk.plan_host(info, np.array([g.nbricks + 5], np.uint32), "column", 4)
     with pytest.raises((BrickError, KeyError)):
         bk.plan_host(info, g.interior, "cube", 4)
+
+
+def _padded(nb):
+    return bk_init_grid([x + 2 for x in nb])
+
+
+def bk_init_grid(dims, periodic=False):
+    from bricklib_amd.bricks import init_grid
+    return init_grid(dims, periodic)
+
+
+@pytest.mark.parametrize("nb", [(3, 2, 4), (5, 7, 3), (1, 1, 1), (4, 4, 9)])
+def test_quad_plans_need_consistent_neighbourhoods(bk, nb):
+    """Interior quads of a padded grid and every-brick quads of a periodic
+    grid build; every-brick quads of a padded (non-periodic) grid, whose
+    outer layer has out-of-range neighbours, are rejected."""
+    from bricklib_amd import BrickError
+    from bricklib_amd.bricks import plan_host
+    g, info = _padded(nb)
+    S0, S1 = nb[0] + 2, nb[1] + 2
+    interior = np.array([(i + 1) + S0 * ((j + 1) + S1 * (k + 1)) for k in range(nb[2])
+                         for j in range(nb[1]) for i in range(nb[0])], np.uint32)
+    for kcol in (1, 4, 8):
+        plan_host(info, interior, "quad", kcol)
+        with pytest.raises(BrickError, match="consistently"):
+            plan_host(info, np.arange(info.nbricks, dtype=np.uint32), "quad", kcol)
+    gp, infop = bk_init_grid(list(nb), periodic=True)
+    for kcol in (1, 8):
+        p = plan_host(infop, np.arange(infop.nbricks, dtype=np.uint32), "quad", kcol)
+        assert np.sort(p["flat"]).tolist() == list(range(infop.nbricks))
+
+
+@pytest.mark.parametrize("nb", [(3, 2, 4), (2, 2, 9)])
+def test_column_links_agree_both_ways(bk, nb):
+    """Column plans over every brick of a padded grid: consecutive bricks
+    of a unit are k-neighbours in both directions, every brick once."""
+    from bricklib_amd.bricks import plan_host
+    g, info = _padded(nb)
+    adj = info.adj
+    for kcol in (2, 8, 64):
+        p = plan_host(info, np.arange(info.nbricks, dtype=np.uint32), "column", kcol)
+        assert np.sort(p["flat"]).tolist() == list(range(info.nbricks))
+        pos = 0
+        for a, ln in zip(p["anchor"], p["len"]):
+            chain = p["flat"][pos:pos + ln]
+            assert chain[0] == a
+            for lo, hi in zip(chain, chain[1:]):
+                assert adj[lo, 22] == hi and adj[hi, 4] == lo
+            pos += ln
