@@ -412,12 +412,12 @@ class PeriodicGrid:
     def auto_plan(self, stencil_kind: int):
         """Default schedule per stencil, from measurements at 512^3 on
         MI355X: 2x2 column quads of 8 levels for the cube, k-marching
-        columns of 8 bricks for the radius-2 and radius-4 stars, and one
-        brick per wave (no plan) for the radius-1 stars."""
-        from ._lib import BK_CUBE125, BK_STAR13, BK_STAR25
+        columns of 8 bricks for the radius-4 star, and one brick per wave
+        (no plan) for the radius-1 and radius-2 stars."""
+        from ._lib import BK_CUBE125, BK_STAR25
         if stencil_kind == BK_CUBE125:
             return self.plan("quad", 8)
-        if stencil_kind in (BK_STAR13, BK_STAR25):
+        if stencil_kind == BK_STAR25:
             return self.plan("column", 8)
         return None
 

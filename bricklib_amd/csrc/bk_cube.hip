@@ -112,7 +112,11 @@ __global__ __launch_bounds__(256) void cube_sweep_lds(const CubeArgs<T> a) {
 // points.  Per output the taps accumulate in (dk, dj, di) lexicographic
 // order, the first as a multiply -- the same arithmetic as the per-brick
 // kernel, so results are identical.
-constexpr int kQuadPW = 20, kQuadCells = kQuadPW * kQuadPW;
+// The padded plane is 20 x 20 cells, stored with a row stride of 24 so the
+// two rows a half-wave's LDS reads touch (2 apart) sit 32 banks apart in
+// both FP64 (16-byte reads) and FP32 (8-byte reads): with stride 20 they
+// overlapped and every read took a bank-conflict pass.
+constexpr int kQuadPW = 20, kQuadStride = 24, kQuadCells = kQuadPW * kQuadStride;
 
 // One input plane P of the quad unit: accumulate its taps into the five
 // rolling slots (slot s holds output plane P - 2 + s, tap dk = 2 - s).
@@ -127,7 +131,7 @@ constexpr int kQuadPW = 20, kQuadCells = kQuadPW * kQuadPW;
 template <class A>
 __device__ __forceinline__ void quad_plane_edge(const A *pl, int lj, int li, kconst_ptr<A> cp,
                                                 int P, int nout, A (&acc)[5][2][2]) {
-  constexpr int PW = kQuadPW;
+  constexpr int PW = kQuadStride;
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
     A x[6];
@@ -186,7 +190,7 @@ template <int G, class A>
 __device__ __forceinline__ void quad_steady_group(const A *pl, int xo, kconst_ptr<A> &cp,
                                                   A (&cf)[2][10], A (&x)[2][6],
                                                   A (&acc)[5][2][2]) {
-  constexpr int PW = kQuadPW;
+  constexpr int PW = kQuadStride;
   typedef A A2 __attribute__((ext_vector_type(2)));
   constexpr int r = G / 5, s = G % 5, cur = G & 1;
   // an accumulator the previous group (rp, sp) wrote
@@ -245,7 +249,7 @@ __device__ __forceinline__ void quad_plane_steady(const A *pl, int lj, int li,
   typedef A A2 __attribute__((ext_vector_type(2)));
   A cf[2][10];
   A x[2][6];
-  const int xo = (2 * lj) * kQuadPW + 2 * li;
+  const int xo = (2 * lj) * kQuadStride + 2 * li;
   asm volatile("" : "+s"(cp));
 #pragma unroll
   for (int q = 0; q < quad_group_count(0); ++q) cf[0][q] = cp[4 * 25 + q];
@@ -270,7 +274,7 @@ __device__ __forceinline__ void cube_quad_unit(
     kconst_ptr<typename acc_of<T>::type> cp0, typename acc_of<T>::type (*plane)[kQuadCells],
     uint32_t *tbl, int lane) {
   using A = typename acc_of<T>::type;
-  constexpr int PW = kQuadPW, NCELL = kQuadCells, NLD = (NCELL + 63) / 64;
+  constexpr int PW = kQuadPW, NCELL = kQuadPW * kQuadPW, NLD = (NCELL + 63) / 64;
   const int lj = lane >> 3, li = lane & 7;
   const uint32_t m1 = adj[(size_t)m0 * 27 + 14];
   const uint32_t m2 = adj[(size_t)m0 * 27 + 16];
@@ -296,7 +300,7 @@ __device__ __forceinline__ void cube_quad_unit(
   }
   __syncthreads();
 
-  int eoff[NLD], tix[NLD];
+  int eoff[NLD], tix[NLD], lofs[NLD];
   bool cv[NLD];
   size_t base[NLD];
 #pragma unroll
@@ -305,6 +309,7 @@ __device__ __forceinline__ void cube_quad_unit(
     cv[t] = q < NCELL;
     const int qq = cv[t] ? q : 0;
     const int J = qq / PW - 2, I = qq % PW - 2;
+    lofs[t] = (J + 2) * kQuadStride + (I + 2);
     tix[t] = ((J + 8) >> 3) * 4 + ((I + 8) >> 3);
     eoff[t] = ((J + 8) & 7) * 8 + ((I + 8) & 7);
     base[t] = (size_t)tbl[tix[t]] * st;
@@ -321,7 +326,7 @@ __device__ __forceinline__ void cube_quad_unit(
   for (int t = 0; t < NLD; ++t) nv[t] = cv[t] ? widen(in[base[t] + 6 * 64 + eoff[t]]) : A(0);
 #pragma unroll
   for (int t = 0; t < NLD; ++t)
-    if (cv[t]) plane[0][t * 64 + lane] = nv[t];
+    if (cv[t]) plane[0][lofs[t]] = nv[t];
   __syncthreads();
 
   A acc[5][2][2];
@@ -388,7 +393,7 @@ __device__ __forceinline__ void cube_quad_unit(
     if (have_next) {
 #pragma unroll
       for (int t = 0; t < NLD; ++t)
-        if (cv[t]) plane[buf ^ 1][t * 64 + lane] = nv[t];
+        if (cv[t]) plane[buf ^ 1][lofs[t]] = nv[t];
     }
     __syncthreads();
   }

@@ -55,6 +55,65 @@ __device__ __forceinline__ A tapv(A x) {
   else return x;
 }
 
+// LDS tile of one brick and its R-deep i/j faces: planes k of P x P cells
+// (P = 8 + 2R), element (k, y, x) at at(k, y, x).  For R = 4 (16-wide rows)
+// bit 3 of x is flipped by bit 1 of y xor the plane's parity: the 32 lanes
+// of a read half-wave touch four consecutive rows of 8 consecutive cells,
+// which then fall on distinct banks (rows y and y+2 would otherwise share
+// them), and so do the face stores below -- the 25-point sweep ran 13%
+// (FP64) and 23% (FP32) faster with it.  R = 1, 2 keep P-wide rows: padding
+// them to 16 for the swizzle cost more in occupancy than it saved.
+template <int R>
+struct Tile {
+  static constexpr int P = 8 + 2 * R;
+  static constexpr bool kSwizzle = R == 4;
+  static constexpr int RS = kSwizzle ? 16 : P;  // row stride
+  static constexpr int PL = P * RS;             // plane stride
+  __device__ static constexpr int at(int k, int y, int x) {
+    return k * PL + y * RS + (kSwizzle ? (x ^ ((((y >> 1) ^ k) & 1) << 3)) : x);
+  }
+};
+
+// The R-deep i/j faces of a brick's four in-plane neighbours, R values per
+// lane per face.  Lane -> (c = lane & 7, rr = (lane >> 3) % R,
+// kq = (lane >> 3) / R); value q belongs to plane q * (8 / R) + kq.  A
+// half-wave then covers whole rows of R-deep face runs: contiguous in
+// memory (R doubles of an i-face row, 8 of a j-face row) and on distinct
+// LDS banks.
+template <class T, int R>
+struct Faces {
+  using A = typename acc_of<T>::type;
+  A jm[R], jp[R], im[R], ip[R];
+
+  __device__ __forceinline__ void load(const T *__restrict__ pjm, const T *__restrict__ pjp,
+                                       const T *__restrict__ pim, const T *__restrict__ pip,
+                                       int lane) {
+    const int c = lane & 7, rr = (lane >> 3) % R, kq = (lane >> 3) / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = q * (8 / R) + kq;
+      jm[q] = widen(pjm[k * 64 + (8 - R + rr) * 8 + c]);
+      jp[q] = widen(pjp[k * 64 + rr * 8 + c]);
+      im[q] = widen(pim[k * 64 + c * 8 + (8 - R + rr)]);
+      ip[q] = widen(pip[k * 64 + c * 8 + rr]);
+    }
+  }
+
+  // into planes k0 .. k0+7 of the tile
+  __device__ __forceinline__ void store(A *tile, int k0, int lane) const {
+    using TL = Tile<R>;
+    const int c = lane & 7, rr = (lane >> 3) % R, kq = (lane >> 3) / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = k0 + q * (8 / R) + kq;
+      tile[TL::at(k, rr, R + c)] = jm[q];
+      tile[TL::at(k, 8 + R + rr, R + c)] = jp[q];
+      tile[TL::at(k, R + c, rr)] = im[q];
+      tile[TL::at(k, R + c, 8 + R + rr)] = ip[q];
+    }
+  }
+};
+
 // One brick of the per-brick schedule, in two phases around a workgroup
 // barrier: load() reads the brick's planes into registers and stages the
 // brick plus its i/j faces in this wave's LDS tile; compute() evaluates and
@@ -66,9 +125,9 @@ struct StarBrick {
   A v[8], vkm[R], vkp[R];
 
   __device__ __forceinline__ void load(const uint32_t *__restrict__ adj, const T *__restrict__ in,
-                                       size_t st, uint32_t b, A (*tile)[P][P], int lane) {
+                                       size_t st, uint32_t b, A *tile, int lane) {
+    using TL = Tile<R>;
     const int j = lane >> 3, i = lane & 7;
-    const int fk = lane >> 3, fl = lane & 7;  // face-load lane mapping
     const uint32_t *ab = adj + (size_t)b * 27;
     const T *pc = in + (size_t)b * st;
     const T *pkm = in + (size_t)uniform(ab[4]) * st;
@@ -84,27 +143,16 @@ struct StarBrick {
       vkm[r] = widen(pkm[(8 - R + r) * 64 + lane]);
       vkp[r] = widen(pkp[r * 64 + lane]);
     }
-    A fjm[R], fjp[R], fim[R], fip[R];
+    Faces<T, R> f;
+    f.load(pjm, pjp, pim, pip, lane);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      fjm[r] = widen(pjm[fk * 64 + (8 - R + r) * 8 + fl]);
-      fjp[r] = widen(pjp[fk * 64 + r * 8 + fl]);
-      fim[r] = widen(pim[fk * 64 + fl * 8 + (8 - R + r)]);
-      fip[r] = widen(pip[fk * 64 + fl * 8 + r]);
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) tile[k][R + j][R + i] = v[k];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      tile[fk][r][R + fl] = fjm[r];
-      tile[fk][8 + R + r][R + fl] = fjp[r];
-      tile[fk][R + fl][r] = fim[r];
-      tile[fk][R + fl][8 + R + r] = fip[r];
-    }
+    for (int k = 0; k < 8; ++k) tile[TL::at(k, R + j, R + i)] = v[k];
+    f.store(tile, 0, lane);
   }
 
   __device__ __forceinline__ void compute(const A (&c)[6 * R + 1], A theta, int cond_mode,
-                                          T *__restrict__ po, A (*tile)[P][P], int lane) {
+                                          T *__restrict__ po, const A *tile, int lane) {
+    using TL = Tile<R>;
     const int j = lane >> 3, i = lane & 7;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -114,10 +162,10 @@ struct StarBrick {
       A acc = c[0] * tapv<A, COND>(v[k]);
 #pragma unroll
       for (int r = 1; r <= R; ++r) {
-        acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[k][R + j][R + i + r]), acc);
-        acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[k][R + j][R + i - r]), acc);
-        acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[k][R + j + r][R + i]), acc);
-        acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[k][R + j - r][R + i]), acc);
+        acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[TL::at(k, R + j, R + i + r)]), acc);
+        acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[TL::at(k, R + j, R + i - r)]), acc);
+        acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[TL::at(k, R + j + r, R + i)]), acc);
+        acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[TL::at(k, R + j - r, R + i)]), acc);
         acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
         acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
       }
@@ -133,8 +181,7 @@ struct StarBrick {
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  constexpr int P = 8 + 2 * R;
-  __shared__ A tile[4][8][P][P];
+  __shared__ A tile[4][8 * Tile<R>::PL];
 
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -165,12 +212,11 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  constexpr int P = 8 + 2 * R;
-  __shared__ A tile[8][P][P];
+  using TL = Tile<R>;
+  __shared__ A tile[8 * TL::PL];
 
   const int lane = threadIdx.x;
   const int j = lane >> 3, i = lane & 7;
-  const int fk = lane >> 3, fl = lane & 7;
   const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
   if (u >= a.nunits) return;
   const uint32_t L = a.len[u];
@@ -194,18 +240,9 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
     // Face loads first: loads retire in order, so the faces can be waited
     // for while the prefetch of the next brick (issued after) stays in
     // flight through the compute below.
-    const T *pjm = a.in + (size_t)ab[10] * st;
-    const T *pjp = a.in + (size_t)ab[16] * st;
-    const T *pim = a.in + (size_t)ab[12] * st;
-    const T *pip = a.in + (size_t)ab[14] * st;
-    A fjm[R], fjp[R], fim[R], fip[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      fjm[r] = widen(pjm[fk * 64 + (8 - R + r) * 8 + fl]);
-      fjp[r] = widen(pjp[fk * 64 + r * 8 + fl]);
-      fim[r] = widen(pim[fk * 64 + fl * 8 + (8 - R + r)]);
-      fip[r] = widen(pip[fk * 64 + fl * 8 + r]);
-    }
+    Faces<T, R> f;
+    f.load(a.in + (size_t)ab[10] * st, a.in + (size_t)ab[16] * st, a.in + (size_t)ab[12] * st,
+           a.in + (size_t)ab[14] * st, lane);
     const T *pu = a.in + (size_t)up * st;
     if (more) {
 #pragma unroll
@@ -215,14 +252,8 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
       for (int r = 0; r < R; ++r) nxt[r] = widen(pu[r * 64 + lane]);
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) tile[k][R + j][R + i] = cur[k];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      tile[fk][r][R + fl] = fjm[r];
-      tile[fk][8 + R + r][R + fl] = fjp[r];
-      tile[fk][R + fl][r] = fim[r];
-      tile[fk][R + fl][8 + R + r] = fip[r];
-    }
+    for (int k = 0; k < 8; ++k) tile[TL::at(k, R + j, R + i)] = cur[k];
+    f.store(tile, 0, lane);
     __syncthreads();
     if (mk[l] & 1u) {
       T *po = a.out + (size_t)b * a.out_step;
@@ -234,10 +265,10 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
         A acc = a.c[0] * tapv<A, COND>(cur[k]);
 #pragma unroll
         for (int r = 1; r <= R; ++r) {
-          acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[k][R + j][R + i + r]), acc);
-          acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[k][R + j][R + i - r]), acc);
-          acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[k][R + j + r][R + i]), acc);
-          acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[k][R + j - r][R + i]), acc);
+          acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[TL::at(k, R + j, R + i + r)]), acc);
+          acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[TL::at(k, R + j, R + i - r)]), acc);
+          acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[TL::at(k, R + j + r, R + i)]), acc);
+          acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[TL::at(k, R + j - r, R + i)]), acc);
           acc = fmadd(a.c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
           acc = fmadd(a.c[6 * r], tapv<A, COND>(X(k - r)), acc);
         }
@@ -269,14 +300,14 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_kstack(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  constexpr int P = 8 + 2 * R, W = 4;
-  __shared__ A tile[W * 8][P][P];
+  using TL = Tile<R>;
+  constexpr int W = 4;
+  __shared__ A tile[W * 8 * TL::PL];
   __shared__ A kb[R][64], ka[R][64];
 
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int j = lane >> 3, i = lane & 7;
-  const int fk = lane >> 3, fl = lane & 7;
   const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
   if (u >= a.nunits) return;  // uniform over the workgroup
   const int L = (int)a.len[u];
@@ -289,20 +320,11 @@ __global__ __launch_bounds__(256) void star_sweep_kstack(const StarArgs<T, R, CO
     b = uniform(a.flat[a.start[u] + (uint32_t)w]);
     const uint32_t *ab = a.adj + (size_t)b * 27;
     const T *pc = a.in + (size_t)b * st;
-    const T *pjm = a.in + (size_t)uniform(ab[10]) * st;
-    const T *pjp = a.in + (size_t)uniform(ab[16]) * st;
-    const T *pim = a.in + (size_t)uniform(ab[12]) * st;
-    const T *pip = a.in + (size_t)uniform(ab[14]) * st;
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = widen(pc[k * 64 + lane]);
-    A fjm[R], fjp[R], fim[R], fip[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      fjm[r] = widen(pjm[fk * 64 + (8 - R + r) * 8 + fl]);
-      fjp[r] = widen(pjp[fk * 64 + r * 8 + fl]);
-      fim[r] = widen(pim[fk * 64 + fl * 8 + (8 - R + r)]);
-      fip[r] = widen(pip[fk * 64 + fl * 8 + r]);
-    }
+    Faces<T, R> f;
+    f.load(a.in + (size_t)uniform(ab[10]) * st, a.in + (size_t)uniform(ab[16]) * st,
+           a.in + (size_t)uniform(ab[12]) * st, a.in + (size_t)uniform(ab[14]) * st, lane);
     if (w == 0) {
       const T *q = a.in + (size_t)uniform(ab[4]) * st;
 #pragma unroll
@@ -314,14 +336,8 @@ __global__ __launch_bounds__(256) void star_sweep_kstack(const StarArgs<T, R, CO
       for (int r = 0; r < R; ++r) ka[r][lane] = widen(q[r * 64 + lane]);
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) tile[w * 8 + k][R + j][R + i] = v[k];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      tile[w * 8 + fk][r][R + fl] = fjm[r];
-      tile[w * 8 + fk][8 + R + r][R + fl] = fjp[r];
-      tile[w * 8 + fk][R + fl][r] = fim[r];
-      tile[w * 8 + fk][R + fl][8 + R + r] = fip[r];
-    }
+    for (int k = 0; k < 8; ++k) tile[TL::at(w * 8 + k, R + j, R + i)] = v[k];
+    f.store(tile, w * 8, lane);
   }
   __syncthreads();
   if (!active || !(a.mask[(size_t)u * a.kcol + w] & 1u)) return;
@@ -335,15 +351,15 @@ __global__ __launch_bounds__(256) void star_sweep_kstack(const StarArgs<T, R, CO
       const int gk = w * 8 + kk;  // plane in the stack
       if (gk < 0) return kb[gk + R][lane];
       if (gk >= top) return ka[gk - top][lane];
-      return tile[gk][R + j][R + i];
+      return tile[TL::at(gk, R + j, R + i)];
     };
     A acc = a.c[0] * tapv<A, COND>(v[k]);
 #pragma unroll
     for (int r = 1; r <= R; ++r) {
-      acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[w * 8 + k][R + j][R + i + r]), acc);
-      acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[w * 8 + k][R + j][R + i - r]), acc);
-      acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[w * 8 + k][R + j + r][R + i]), acc);
-      acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[w * 8 + k][R + j - r][R + i]), acc);
+      acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[TL::at(w * 8 + k, R + j, R + i + r)]), acc);
+      acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[TL::at(w * 8 + k, R + j, R + i - r)]), acc);
+      acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[TL::at(w * 8 + k, R + j + r, R + i)]), acc);
+      acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[TL::at(w * 8 + k, R + j - r, R + i)]), acc);
       acc = fmadd(a.c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
       acc = fmadd(a.c[6 * r], tapv<A, COND>(X(k - r)), acc);
     }
@@ -404,8 +420,7 @@ hipError_t launch_cols_dt(const SweepLaunch &L, const PlanView &P, hipStream_t s
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_batched(const BatchTable *tab_, uint32_t tick) {
   using A = typename acc_of<T>::type;
-  constexpr int P = 8 + 2 * R;
-  __shared__ A tile[4][8][P][P];
+  __shared__ A tile[4][8 * Tile<R>::PL];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
