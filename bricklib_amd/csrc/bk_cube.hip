@@ -132,15 +132,29 @@ template <class A>
 __device__ __forceinline__ void quad_plane_edge(const A *pl, int lj, int li, kconst_ptr<A> cp,
                                                 int P, int nout, A (&acc)[5][2][2]) {
   constexpr int PW = kQuadStride;
+  int ro = (2 * lj) * PW + 2 * li;
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
     A x[6];
+    if (r > 0) {
+      // row r's reads wait for row r-1's FMAs (every accumulator is an
+      // input): left alone, the scheduler issues all six rows' reads up
+      // front, and their 72 live registers cost the FP64 kernel a wave
+      ro += PW;
+      asm volatile(""
+                   : "+v"(ro)
+                   : "v"(acc[0][0][0]), "v"(acc[0][0][1]), "v"(acc[0][1][0]), "v"(acc[0][1][1]),
+                     "v"(acc[1][0][0]), "v"(acc[1][0][1]), "v"(acc[1][1][0]), "v"(acc[1][1][1]),
+                     "v"(acc[2][0][0]), "v"(acc[2][0][1]), "v"(acc[2][1][0]), "v"(acc[2][1][1]),
+                     "v"(acc[3][0][0]), "v"(acc[3][0][1]), "v"(acc[3][1][0]), "v"(acc[3][1][1]),
+                     "v"(acc[4][0][0]), "v"(acc[4][0][1]), "v"(acc[4][1][0]), "v"(acc[4][1][1]));
+    }
     // 16-byte LDS reads of the lane's two adjacent columns: 8 lanes of a
     // row cover 128 contiguous bytes, so a read is free of bank conflicts
 #pragma unroll
     for (int c = 0; c < 6; c += 2) {
       typedef A A2 __attribute__((ext_vector_type(2)));
-      const A2 v = *reinterpret_cast<const A2 *>(&pl[(2 * lj + r) * PW + 2 * li + c]);
+      const A2 v = *reinterpret_cast<const A2 *>(&pl[ro + c]);
       x[c] = v.x;
       x[c + 1] = v.y;
     }
@@ -300,20 +314,35 @@ __device__ __forceinline__ void cube_quad_unit(
   }
   __syncthreads();
 
-  int eoff[NLD], tix[NLD], lofs[NLD];
-  bool cv[NLD];
-  size_t base[NLD];
+  // Load t of a plane fetches cell q = 64t + lane of the 20x20 padded plane
+  // (the last load only for the first NCELL - 64(NLD-1) lanes).  Its only
+  // state across planes is one 32-bit element offset into the input field,
+  // brick * step + in-brick offset (the launchers check the field fits 2^32
+  // elements), recomputed from the brick table at each new input level --
+  // 7 registers where the brick bases, offsets, table and LDS indices took
+  // 35, which is what lets the FP64 kernel hold a fifth wave per SIMD.
+  const bool tail = lane < NCELL - 64 * (NLD - 1);
+  const uint32_t st32 = (uint32_t)st;
+  uint32_t off[NLD];
+  auto set_offsets = [&]() {
+    // the lane index through an empty asm: the compiler cannot hoist the
+    // per-load cell arithmetic out of the plane loop and keep it live
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const bool tl = ln < NCELL - 64 * (NLD - 1);
 #pragma unroll
-  for (int t = 0; t < NLD; ++t) {
+    for (int t = 0; t < NLD; ++t) {
+      const int q = t * 64 + ln;
+      const int qq = (t < NLD - 1 || tl) ? q : 0;
+      const int J = qq / PW + 6, I = qq % PW + 6;  // cell + 8 within the 4x4 block
+      off[t] = tbl[(J >> 3) * 4 + (I >> 3)] * st32 + (uint32_t)((J & 7) * 8 + (I & 7));
+    }
+  };
+  auto lds_cell = [&](int t) {
     const int q = t * 64 + lane;
-    cv[t] = q < NCELL;
-    const int qq = cv[t] ? q : 0;
-    const int J = qq / PW - 2, I = qq % PW - 2;
-    lofs[t] = (J + 2) * kQuadStride + (I + 2);
-    tix[t] = ((J + 8) >> 3) * 4 + ((I + 8) >> 3);
-    eoff[t] = ((J + 8) & 7) * 8 + ((I + 8) & 7);
-    base[t] = (size_t)tbl[tix[t]] * st;
-  }
+    return q + (q / PW) * (kQuadStride - PW);
+  };
+  set_offsets();
   uint32_t tn = 0;  // next level's table entry (lanes 0..15)
   if (lane < 16) tn = tbl[16 + lane];
 
@@ -323,10 +352,11 @@ __device__ __forceinline__ void cube_quad_unit(
 
   A nv[NLD];
 #pragma unroll
-  for (int t = 0; t < NLD; ++t) nv[t] = cv[t] ? widen(in[base[t] + 6 * 64 + eoff[t]]) : A(0);
+  for (int t = 0; t < NLD; ++t)
+    nv[t] = (t < NLD - 1 || tail) ? widen(in[(size_t)off[t] + 6 * 64]) : A(0);
 #pragma unroll
   for (int t = 0; t < NLD; ++t)
-    if (cv[t]) plane[0][lofs[t]] = nv[t];
+    if (t < NLD - 1 || tail) plane[0][lds_cell(t)] = nv[t];
   __syncthreads();
 
   A acc[5][2][2];
@@ -348,14 +378,13 @@ __device__ __forceinline__ void cube_quad_unit(
       if ((Pn & 7) == 0) {  // next plane starts a new input level
         if (lane < 16) tbl[lane] = tn;
         __syncthreads();
-#pragma unroll
-        for (int t = 0; t < NLD; ++t) base[t] = (size_t)tbl[tix[t]] * st;
+        set_offsets();
         if (lane < 16) tn = adj[(size_t)tbl[lane] * 27 + 22];
       }
       const int kp = Pn & 7;
 #pragma unroll
       for (int t = 0; t < NLD; ++t)
-        nv[t] = cv[t] ? widen(in[base[t] + kp * 64 + eoff[t]]) : A(0);
+        nv[t] = (t < NLD - 1 || tail) ? widen(in[(size_t)off[t] + kp * 64]) : A(0);
     }
 
     const A *pl = plane[buf];
@@ -393,7 +422,7 @@ __device__ __forceinline__ void cube_quad_unit(
     if (have_next) {
 #pragma unroll
       for (int t = 0; t < NLD; ++t)
-        if (cv[t]) plane[buf ^ 1][lofs[t]] = nv[t];
+        if (t < NLD - 1 || tail) plane[buf ^ 1][lds_cell(t)] = nv[t];
     }
     __syncthreads();
   }
