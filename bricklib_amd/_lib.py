@@ -103,6 +103,9 @@ SIGNATURES = {
     "bk_sweeps_deep": (C.c_int, [C.POINTER(bk_sweep_desc), vp, C.c_uint32, vp, vp, C.c_uint32,
                                  C.c_int, C.c_int, C.c_int, vp]),
     "bk_sweeps": (C.c_int, [C.POINTER(bk_sweep_desc), vp, C.c_uint32, C.c_int, C.c_int, vp]),
+    "bk_sweeps_fused": (C.c_int, [C.POINTER(bk_sweep_desc), vp, vp, vp, C.c_uint32, C.c_int,
+                                  C.c_int, vp]),
+    "bk_ghost_images": (C.c_int, [longp, u32p, u32p, C.POINTER(C.c_uint32)]),
     "bk_graph_cache_clear": (C.c_int, []),
     "bk_fill_uniform": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_uint64, C.c_double,
                                   C.c_double, vp]),

@@ -402,6 +402,20 @@ class PeriodicGrid:
         return self._plans[key]
 
     @property
+    def ghost_images(self):
+        """(d_img_off, d_img): the ghost images of every brick as CSR in HBM
+        (bk_ghost_images), for the fused refresh; built on first use."""
+        if getattr(self, "_images", None) is None:
+            n = C.c_uint32()
+            check(load().bk_ghost_images(longs(self.nb), None, None, C.byref(n)))
+            off = np.zeros(self.nbricks + 1, np.uint32)
+            img = np.zeros(max(1, n.value), np.uint32)
+            check(load().bk_ghost_images(longs(self.nb), off.ctypes.data_as(_lib.u32p),
+                                         img.ctypes.data_as(_lib.u32p), C.byref(n)))
+            self._images = (DeviceBuffer.from_array(off), DeviceBuffer.from_array(img))
+        return self._images
+
+    @property
     def d_every(self) -> DeviceBuffer:
         """Every brick of the padded grid in grid order, in HBM (the sweep
         list of brick_stencil(..., skip = 0), weak/main.cpp:26-36)."""
@@ -524,17 +538,26 @@ def sweep(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, stream=None,
 
 def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
            use_graph: bool = False, refresh: bool = True, stream=None, plan="auto",
-           ghost_iters: int = 1) -> Brick:
+           ghost_iters: int = 1, fused: bool = False) -> Brick:
     """``nsweeps`` ping-pong sweeps a->b->a..., each preceded by the periodic
     ghost refresh of its input -- or, with ghost_iters > 1, the refresh once
     per ghost_iters sweeps and the sweeps in between over every brick of the
-    padded grid (bk_sweeps_deep; same result).  Returns the field holding
-    the result."""
+    padded grid (bk_sweeps_deep), or, with fused=True, the refresh folded
+    into the sweeps' stores (bk_sweeps_fused; stars and cond, one brick per
+    wave).  Same result in every form.  Returns the field holding the
+    result."""
     keep: list = []
     d = _desc(st, b, a, grid, keep, plan=plan)
     pairs = grid.d_pairs.ptr if (refresh and grid.d_pairs is not None) else None
     npairs = grid.npairs if refresh else 0
-    if ghost_iters == 1:
+    if fused:
+        if ghost_iters != 1:
+            raise ValueError("the fused refresh refreshes every sweep (ghost_iters = 1)")
+        io, im = grid.ghost_images
+        check(load().bk_sweeps_fused(C.byref(d), _vp(io.ptr), _vp(im.ptr), _vp(pairs),
+                                     C.c_uint32(npairs), int(nsweeps), int(bool(use_graph)),
+                                     _vp(_stream_of(stream))))
+    elif ghost_iters == 1:
         check(load().bk_sweeps(C.byref(d), _vp(pairs), C.c_uint32(npairs), int(nsweeps),
                                int(bool(use_graph)), _vp(_stream_of(stream))))
     else:

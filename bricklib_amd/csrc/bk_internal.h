@@ -24,6 +24,10 @@ struct SweepLaunch {
   int ncoeff;
   double theta;
   int cond_mode;
+  // fused refresh (per-brick star kernel only): the ghost images of brick b
+  // are img[img_off[b] .. img_off[b+1]); each also receives b's output
+  const uint32_t *img_off = nullptr;
+  const uint32_t *img = nullptr;
 };
 
 // Device view of an execution plan (see bk_plan.hip).

@@ -11,6 +11,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "bk_api.h"
@@ -153,9 +154,14 @@ hipError_t launch_sweep(const bk::SweepLaunch &L, const bk_plan_s *plan, hipStre
 // included -- otherwise.  ghost_iters == 1 is the refresh-every-sweep loop.
 hipError_t enqueue_sweeps(const bk_sweep_desc *d, const uint32_t *pairs, uint32_t npairs,
                           int nsweeps, int ghost_iters, const uint32_t *all, uint32_t nall,
-                          bk_plan all_plan, hipStream_t s) {
+                          bk_plan all_plan, const uint32_t *img_off, const uint32_t *img,
+                          hipStream_t s) {
   bk::SweepLaunch L;
   to_launch(d, L);
+  // fused refresh: the input's ghosts once, then every sweep also writes
+  // its output into the output's ghost bricks
+  L.img_off = img_off;
+  L.img = img;
   const size_t es = (size_t)dt_size(d->dtype);
   for (int t = 0; t < nsweeps; ++t) {
     const bool fwd = (t % 2) == 0;
@@ -163,7 +169,7 @@ hipError_t enqueue_sweeps(const bk_sweep_desc *d, const uint32_t *pairs, uint32_
     void *dst = fwd ? d->d_out : const_cast<void *>(d->d_in);
     const size_t sst = fwd ? d->in_step : d->out_step;
     const size_t dstep = fwd ? d->out_step : d->in_step;
-    if (npairs && t % ghost_iters == 0) {
+    if (npairs && (img_off ? t == 0 : t % ghost_iters == 0)) {
       hipError_t e = bk::launch_ghost_refresh(src, sst * es, 512 * es, pairs, npairs, s);
       if (e != hipSuccess) return e;
     }
@@ -472,6 +478,28 @@ int bk_ghost_pairs(const long nb[3], uint32_t *h_pairs, uint32_t *npairs) {
   return BK_OK;
 }
 
+int bk_ghost_images(const long nb[3], uint32_t *h_off, uint32_t *h_img, uint32_t *nimg) {
+  if (!nb || !nimg) return fail(BK_ERR_ARG, "bk_ghost_images: null pointer");
+  uint32_t np = 0;
+  int rc = bk_ghost_pairs(nb, nullptr, &np);
+  if (rc != BK_OK) return rc;
+  *nimg = np;
+  if (!h_off && !h_img) return BK_OK;
+  if (!h_off || !h_img) return fail(BK_ERR_ARG, "bk_ghost_images: null output array");
+  long S[3];
+  grid_dims(nb, S);
+  const size_t nbricks = (size_t)S[0] * S[1] * S[2];
+  std::vector<uint32_t> pairs(2 * (size_t)np);
+  bk_ghost_pairs(nb, pairs.data(), &np);
+  // counting sort of the ghost bricks by their periodic image
+  std::fill(h_off, h_off + nbricks + 1, 0u);
+  for (uint32_t q = 0; q < np; ++q) ++h_off[pairs[2 * q + 1] + 1];
+  for (size_t b = 0; b < nbricks; ++b) h_off[b + 1] += h_off[b];
+  std::vector<uint32_t> fillp(h_off, h_off + nbricks);
+  for (uint32_t q = 0; q < np; ++q) h_img[fillp[pairs[2 * q + 1]]++] = pairs[2 * q];
+  return BK_OK;
+}
+
 int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n) {
   if (!nb || !n) return fail(BK_ERR_ARG, "bk_interior_list: null pointer");
   for (int a = 0; a < 3; ++a)
@@ -519,10 +547,11 @@ namespace {
 
 int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, int nsweeps,
                 int ghost_iters, const uint32_t *d_all, uint32_t nall, bk_plan all_plan,
-                int use_graph, hipStream_t s) {
+                const uint32_t *img_off, const uint32_t *img, int use_graph, hipStream_t s) {
   if (nsweeps == 0 || desc_empty(d)) return BK_OK;
   if (!use_graph) {
-    BK_HIP(enqueue_sweeps(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan, s));
+    BK_HIP(enqueue_sweeps(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan,
+                          img_off, img, s));
     return BK_OK;
   }
   // key: every argument that shapes the captured launches
@@ -551,6 +580,8 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
   put(&d_all, sizeof(d_all));
   put(&nall, sizeof(nall));
   put(&all_plan, sizeof(all_plan));
+  put(&img_off, sizeof(img_off));
+  put(&img, sizeof(img));
   auto &gc = graphs();
   std::lock_guard<std::mutex> lock(gc.mu);
   auto it = gc.exec.find(key);
@@ -568,8 +599,8 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
       cs = cit->second;
     }
     BK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    hipError_t e =
-        enqueue_sweeps(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan, cs);
+    hipError_t e = enqueue_sweeps(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall,
+                                  all_plan, img_off, img, cs);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e != hipSuccess) {
@@ -595,8 +626,24 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, 
   if (rc != BK_OK) return rc;
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps: null pair list");
-  return sweeps_impl(d, d_pairs, npairs, nsweeps, 1, d->d_list, d->nlist, d->plan, use_graph,
-                     S(s));
+  return sweeps_impl(d, d_pairs, npairs, nsweeps, 1, d->d_list, d->nlist, d->plan, nullptr,
+                     nullptr, use_graph, S(s));
+}
+
+int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off, const uint32_t *d_img,
+                    const uint32_t *d_pairs, uint32_t npairs, int nsweeps, int use_graph,
+                    bk_stream s) {
+  int rc = check_desc(d);
+  if (rc != BK_OK) return rc;
+  if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps_fused: negative sweep count");
+  if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps_fused: null pair list");
+  if (!d_img_off || !d_img) return fail(BK_ERR_ARG, "bk_sweeps_fused: null ghost-image table");
+  if (d->stencil == BK_CUBE125 || d->plan)
+    return fail(BK_ERR_UNSUPPORTED,
+                "bk_sweeps_fused: the fused refresh runs in the per-brick star/cond kernel "
+                "(no plan, no cube); use bk_sweeps");
+  return sweeps_impl(d, d_pairs, npairs, nsweeps, 1, d->d_list, d->nlist, d->plan, d_img_off,
+                     d_img, use_graph, S(s));
 }
 
 int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
@@ -618,8 +665,8 @@ int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
   all.plan = all_plan;
   rc = check_desc(&all);
   if (rc != BK_OK) return rc;
-  return sweeps_impl(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan,
-                     use_graph, S(s));
+  return sweeps_impl(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan, nullptr,
+                     nullptr, use_graph, S(s));
 }
 
 int bk_graph_cache_clear(void) {

@@ -45,6 +45,9 @@ struct StarArgs {
   const uint32_t *__restrict__ start;
   uint32_t nunits;
   uint32_t kcol;
+  // fused refresh: ghost images of brick b, img[img_off[b] .. img_off[b+1])
+  const uint32_t *__restrict__ img_off;
+  const uint32_t *__restrict__ img;
   A theta;
   A c[6 * R + 1];
 };
@@ -150,8 +153,15 @@ struct StarBrick {
     f.store(tile, 0, lane);
   }
 
+  // out: the output field, ost its step; b's 512 points go to brick b and
+  // to its nimg ghost images imgs[0..nimg) (the fused refresh; nimg = 0
+  // without it)
   __device__ __forceinline__ void compute(const A (&c)[6 * R + 1], A theta, int cond_mode,
-                                          T *__restrict__ po, const A *tile, int lane) {
+                                          T *__restrict__ out, size_t ost, uint32_t b,
+                                          const A *tile, int lane,
+                                          const uint32_t *__restrict__ imgs = nullptr,
+                                          uint32_t nimg = 0) {
+    T *__restrict__ po = out + (size_t)b * ost;
     using TL = Tile<R>;
     const int j = lane >> 3, i = lane & 7;
 #pragma unroll
@@ -174,6 +184,8 @@ struct StarBrick {
         acc = sel ? acc : -acc;
       }
       store_nt(&po[k * 64 + lane], acc);
+      for (uint32_t q = 0; q < nimg; ++q)
+        store_nt(&out[(size_t)uniform(imgs[q]) * ost + k * 64 + lane], acc);
     }
   }
 };
@@ -199,7 +211,14 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
   A c[6 * R + 1];
 #pragma unroll
   for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
-  sb.compute(c, a.theta, a.cond_mode, a.out + (size_t)b * a.out_step, tile[w], lane);
+  const uint32_t *imgs = nullptr;
+  uint32_t nimg = 0;
+  if (a.img_off) {
+    const uint32_t i0 = uniform(a.img_off[b]);
+    nimg = uniform(a.img_off[b + 1]) - i0;
+    imgs = a.img + i0;
+  }
+  sb.compute(c, a.theta, a.cond_mode, a.out, a.out_step, b, tile[w], lane, imgs, nimg);
 }
 
 
@@ -390,6 +409,7 @@ hipError_t launch_cols_t(const SweepLaunch &L, const PlanView &P, hipStream_t s)
   a.start = P.start;
   a.nunits = P.nunits;
   a.kcol = P.kcol;
+  a.img_off = a.img = nullptr;
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
   if (P.nunits == 0) return hipSuccess;
@@ -456,7 +476,7 @@ __global__ __launch_bounds__(256) void star_sweep_batched(const BatchTable *tab_
       A c[6 * R + 1];
 #pragma unroll
       for (int q = 0; q < 6 * R + 1; ++q) c[q] = cp[q];
-      sb.compute(c, (A)sg.theta, sg.cond_mode, out + (size_t)b * sg.step, tile[w], lane);
+      sb.compute(c, (A)sg.theta, sg.cond_mode, out, sg.step, b, tile[w], lane);
     }
     __syncthreads();
   }
@@ -500,6 +520,8 @@ hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   a.nlist = L.nlist;
   a.cond_mode = L.cond_mode;
   a.anchor = a.len = a.mask = a.flat = a.start = nullptr;
+  a.img_off = L.img_off;
+  a.img = L.img;
   a.nunits = a.kcol = 0;
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];

@@ -160,6 +160,13 @@ int bk_init_grid(const long dims[3], int periodic, uint32_t *h_grid,
  * Cartesian communicator of weak/args.cpp:107-108).  h_pairs may be NULL to
  * query *npairs. */
 int bk_ghost_pairs(const long nb[3], uint32_t *h_pairs, uint32_t *npairs);
+/* The inverse of bk_ghost_pairs: for every brick b of the padded grid
+ * ((nb+2)^3 numbered as bk_init_grid), the ghost bricks whose periodic image
+ * it is, as CSR -- h_img[h_off[b] .. h_off[b+1]) -- in pair order.  h_off
+ * has nbricks + 1 entries, h_img *nimg (= the pair count).  Both NULL to
+ * query *nimg.  Feeds the fused refresh of bk_sweeps_fused. */
+int bk_ghost_images(const long nb[3], uint32_t *h_off, uint32_t *h_img,
+                    uint32_t *nimg);
 /* Interior brick list in grid order (k slowest, i fastest) for the same
  * padded grid: the bricks the sweep loops of stencils/3axis.cpp:51-66 visit.
  * h_list may be NULL to query *n. */
@@ -267,6 +274,16 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs,
 int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
                    bk_plan all_plan, const uint32_t *d_pairs, uint32_t npairs,
                    int nsweeps, int ghost_iters, int use_graph, bk_stream s);
+/* bk_sweeps with the refresh fused into the sweep: the ghost bricks of
+ * d_in are refreshed once (d_pairs), then every sweep also stores each
+ * output brick into its ghost images (d_img_off / d_img from
+ * bk_ghost_images, uploaded), so the next sweep's input is already
+ * refreshed.  The same bits as bk_sweeps, with one ghost write per sweep
+ * instead of a refresh's read + write.  Star and cond stencils without a
+ * plan (one brick per wave); otherwise BK_ERR_UNSUPPORTED. */
+int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off,
+                    const uint32_t *d_img, const uint32_t *d_pairs, uint32_t npairs,
+                    int nsweeps, int use_graph, bk_stream s);
 /* Drop every cached graph (frees their device resources). */
 int bk_graph_cache_clear(void);
 

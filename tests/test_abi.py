@@ -119,3 +119,39 @@ def test_python_mirror_raises(bk):
     from bricklib_amd import BrickError
     with pytest.raises(BrickError):
         bk.BrickStorage.allocate(-1, 512)
+
+
+@pytest.mark.parametrize("nb", [(1, 1, 1), (3, 2, 4), (5, 5, 5)])
+def test_ghost_images_invert_pairs(bk, nb):
+    from bricklib_amd import _lib
+    from bricklib_amd._lib import longs
+    lib = bk.load()
+    n = C.c_uint32()
+    assert lib.bk_ghost_images(longs(nb), None, None, C.byref(n)) == 0
+    S = [x + 2 for x in nb]
+    N = S[0] * S[1] * S[2]
+    off = np.zeros(N + 1, np.uint32)
+    img = np.zeros(n.value, np.uint32)
+    assert lib.bk_ghost_images(longs(nb), off.ctypes.data_as(_lib.u32p),
+                               img.ctypes.data_as(_lib.u32p), C.byref(n)) == 0
+    pairs = np.zeros(2 * n.value, np.uint32)
+    lib.bk_ghost_pairs(longs(nb), pairs.ctypes.data_as(_lib.u32p), C.byref(n))
+    pairs = pairs.reshape(-1, 2)
+    for b in range(N):
+        assert sorted(img[off[b]:off[b + 1]].tolist()) == sorted(pairs[pairs[:, 1] == b, 0].tolist())
+    assert off[-1] == n.value
+
+
+def test_fused_sweeps_argument_errors(bk):
+    from bricklib_amd._lib import BK_ERR_ARG, BK_ERR_UNSUPPORTED, bk_sweep_desc
+    lib = bk.load()
+    d = bk_sweep_desc()
+    coeff = np.ones(125)
+    d.h_coeff = coeff.ctypes.data_as(C.POINTER(C.c_double))
+    d.nlist = 4
+    d.d_adj = d.d_list = d.d_in = 4096
+    d.d_out = 8192
+    d.in_step = d.out_step = 512
+    assert lib.bk_sweeps_fused(C.byref(d), None, None, None, 0, 2, 0, None) == BK_ERR_ARG
+    d.stencil = 3  # the cube has no fused refresh
+    assert lib.bk_sweeps_fused(C.byref(d), 64, 128, None, 0, 2, 0, None) == BK_ERR_UNSUPPORTED

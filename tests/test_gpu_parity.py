@@ -317,3 +317,42 @@ def test_deep_ghost_rejects_too_many_sweeps(gpu):
     b = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
     with pytest.raises(BrickError):
         gpu.sweeps(st, a, b, g, 4, ghost_iters=3)
+
+
+@pytest.mark.parametrize("name", ["7pt", "13pt", "25pt", "cond"])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_fused_refresh_matches_refresh_every_sweep(gpu, name, use_graph):
+    """bk_sweeps_fused (ghosts refreshed once, then written by each sweep's
+    stores) gives the refresh-every-sweep result bit for bit, for odd and
+    even sweep counts, on grids down to one brick (26 ghost images)."""
+    from bricklib_amd import BK_BF16, BK_F32
+    st = S.Stencil.make(name, seed=31)
+    for nb in ((3, 2, 4), (1, 1, 1), (2, 5, 1)):
+        for dt in (BK_F64, BK_F32, BK_BF16):
+            g = gpu.PeriodicGrid(nb)
+            for n in (1, 4, 5):
+                outs = []
+                for fused in (False, True):
+                    a = gpu.Brick(g.info, g.field_storage(dt), 0)
+                    b = gpu.Brick(g.info, g.field_storage(dt), 0)
+                    g.fill_uniform(a, 6)
+                    res = gpu.sweeps(st, a, b, g, n, use_graph=use_graph, plan=None,
+                                     fused=fused)
+                    outs.append(res.storage.to_host())
+                print(f"[case] fused {name} nb={nb} dt={dt} n={n} graph={use_graph}", flush=True)
+                np.testing.assert_array_equal(outs[1][g.interior], outs[0][g.interior])
+                # the fused form leaves the result's ghosts refreshed too
+                pr = g.pairs.reshape(-1, 2)
+                np.testing.assert_array_equal(outs[1][pr[:, 0]], outs[1][pr[:, 1]])
+
+
+def test_fused_refresh_rejects_plans_and_cube(gpu):
+    from bricklib_amd import BrickError
+    g = gpu.PeriodicGrid((2, 2, 2))
+    a = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    b = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    with pytest.raises(BrickError):
+        gpu.sweeps(S.Stencil.make("125pt", seed=1), a, b, g, 2, fused=True)
+    with pytest.raises(BrickError):
+        gpu.sweeps(S.Stencil.make("25pt", seed=1), a, b, g, 2, fused=True,
+                   plan=g.plan("column", 8))
