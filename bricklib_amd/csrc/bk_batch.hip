@@ -173,6 +173,9 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
       if (!q.plan || bk_plan_query(q.plan, nullptr, nullptr, &shape, nullptr) != BK_OK ||
           shape != BK_PLAN_QUAD)
         return fail(BK_ERR_ARG, "bk_batch_set: a cube request needs a BK_PLAN_QUAD plan");
+      if (!bk::quad_offsets_fit(bk::plan_view(q.plan).nbricks, q.step))
+        return fail(BK_ERR_UNSUPPORTED,
+                    "bk_batch_set: the quad schedule addresses at most 2^32 input elements");
     } else if (q.nlist && !q.d_list) {
       return fail(BK_ERR_ARG, "bk_batch_set: null brick list");
     }

@@ -118,6 +118,13 @@ __global__ __launch_bounds__(256) void cube_sweep_lds(const CubeArgs<T> a) {
 // overlapped and every read took a bank-conflict pass.
 constexpr int kQuadPW = 20, kQuadStride = 24, kQuadCells = kQuadPW * kQuadStride;
 
+// Two adjacent elements read as one load (16 bytes in FP64, 8 in FP32, 4 in
+// BF16); only ever at an even element index of a brick row.
+template <class T>
+struct alignas(2 * sizeof(T)) Pair {
+  T a, b;
+};
+
 // One input plane P of the quad unit: accumulate its taps into the five
 // rolling slots (slot s holds output plane P - 2 + s, tap dk = 2 - s).
 // Taps go row by row of the LDS plane (r), then slot by slot; per output
@@ -288,7 +295,9 @@ __device__ __forceinline__ void cube_quad_unit(
     kconst_ptr<typename acc_of<T>::type> cp0, typename acc_of<T>::type (*plane)[kQuadCells],
     uint32_t *tbl, int lane) {
   using A = typename acc_of<T>::type;
-  constexpr int PW = kQuadPW, NCELL = kQuadPW * kQuadPW, NLD = (NCELL + 63) / 64;
+  // the padded plane in 2-cell chunks: a 20-cell row is 10 chunks, each
+  // inside one brick row and 2-cell aligned there (16 bytes in FP64)
+  constexpr int PW = kQuadPW, CPR = kQuadPW / 2, NCH = kQuadPW * CPR, NLD = (NCH + 63) / 64;
   const int lj = lane >> 3, li = lane & 7;
   const uint32_t m1 = adj[(size_t)m0 * 27 + 14];
   const uint32_t m2 = adj[(size_t)m0 * 27 + 16];
@@ -314,14 +323,14 @@ __device__ __forceinline__ void cube_quad_unit(
   }
   __syncthreads();
 
-  // Load t of a plane fetches cell q = 64t + lane of the 20x20 padded plane
-  // (the last load only for the first NCELL - 64(NLD-1) lanes).  Its only
+  // Load t of a plane fetches chunk q = 64t + lane of the 20x20 padded
+  // plane, two cells in one load (the last load only for the first
+  // NCH - 64(NLD-1) lanes): four loads per plane instead of seven.  Its only
   // state across planes is one 32-bit element offset into the input field,
-  // brick * step + in-brick offset (the launchers check the field fits 2^32
-  // elements), recomputed from the brick table at each new input level --
-  // 7 registers where the brick bases, offsets, table and LDS indices took
-  // 35, which is what lets the FP64 kernel hold a fifth wave per SIMD.
-  const bool tail = lane < NCELL - 64 * (NLD - 1);
+  // brick * step + in-brick offset (bk_plan_create and bk_batch_set reject
+  // fields beyond 2^32 elements for the quad schedule), recomputed from the
+  // brick table at each new input level.
+  const bool tail = lane < NCH - 64 * (NLD - 1);
   const uint32_t st32 = (uint32_t)st;
   uint32_t off[NLD];
   auto set_offsets = [&]() {
@@ -329,18 +338,26 @@ __device__ __forceinline__ void cube_quad_unit(
     // per-load cell arithmetic out of the plane loop and keep it live
     int ln = lane;
     asm volatile("" : "+v"(ln));
-    const bool tl = ln < NCELL - 64 * (NLD - 1);
+    const bool tl = ln < NCH - 64 * (NLD - 1);
 #pragma unroll
     for (int t = 0; t < NLD; ++t) {
       const int q = t * 64 + ln;
       const int qq = (t < NLD - 1 || tl) ? q : 0;
-      const int J = qq / PW + 6, I = qq % PW + 6;  // cell + 8 within the 4x4 block
+      const int J = qq / CPR + 6, I = (qq % CPR) * 2 + 6;  // cell + 8 in the 4x4 block
       off[t] = tbl[(J >> 3) * 4 + (I >> 3)] * st32 + (uint32_t)((J & 7) * 8 + (I & 7));
     }
   };
   auto lds_cell = [&](int t) {
     const int q = t * 64 + lane;
-    return q + (q / PW) * (kQuadStride - PW);
+    return (q / CPR) * kQuadStride + (q % CPR) * 2;
+  };
+  typedef A A2 __attribute__((ext_vector_type(2)));
+  auto load2 = [&](uint32_t e) -> A2 {
+    const Pair<T> v = *reinterpret_cast<const Pair<T> *>(in + e);
+    A2 r;
+    r.x = widen(v.a);
+    r.y = widen(v.b);
+    return r;
   };
   set_offsets();
   uint32_t tn = 0;  // next level's table entry (lanes 0..15)
@@ -350,13 +367,13 @@ __device__ __forceinline__ void cube_quad_unit(
   const int jo = (2 * lj) & 7, io = (2 * li) & 7;
   uint32_t om0 = m0, om1 = m1, om2 = m2, om3 = m3;
 
-  A nv[NLD];
+  A2 nv[NLD];
 #pragma unroll
   for (int t = 0; t < NLD; ++t)
-    nv[t] = (t < NLD - 1 || tail) ? widen(in[(size_t)off[t] + 6 * 64]) : A(0);
+    if (t < NLD - 1 || tail) nv[t] = load2(off[t] + 6 * 64);
 #pragma unroll
   for (int t = 0; t < NLD; ++t)
-    if (t < NLD - 1 || tail) plane[0][lds_cell(t)] = nv[t];
+    if (t < NLD - 1 || tail) *reinterpret_cast<A2 *>(&plane[0][lds_cell(t)]) = nv[t];
   __syncthreads();
 
   A acc[5][2][2];
@@ -384,7 +401,7 @@ __device__ __forceinline__ void cube_quad_unit(
       const int kp = Pn & 7;
 #pragma unroll
       for (int t = 0; t < NLD; ++t)
-        nv[t] = (t < NLD - 1 || tail) ? widen(in[(size_t)off[t] + kp * 64]) : A(0);
+        if (t < NLD - 1 || tail) nv[t] = load2(off[t] + kp * 64);
     }
 
     const A *pl = plane[buf];
@@ -422,7 +439,7 @@ __device__ __forceinline__ void cube_quad_unit(
     if (have_next) {
 #pragma unroll
       for (int t = 0; t < NLD; ++t)
-        if (t < NLD - 1 || tail) plane[buf ^ 1][lds_cell(t)] = nv[t];
+        if (t < NLD - 1 || tail) *reinterpret_cast<A2 *>(&plane[buf ^ 1][lds_cell(t)]) = nv[t];
     }
     __syncthreads();
   }
@@ -431,7 +448,7 @@ __device__ __forceinline__ void cube_quad_unit(
 template <class T>
 __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
   using A = typename acc_of<T>::type;
-  __shared__ A plane[2][kQuadCells];
+  __shared__ __attribute__((aligned(16))) A plane[2][kQuadCells];
   __shared__ uint32_t tbl[32];
   const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
   if (u >= a.nunits) return;
@@ -451,7 +468,7 @@ template <class T>
 __global__ __launch_bounds__(64, sizeof(T) == 8 ? 4 : 1) void cube_sweep_batched(const BatchTable *tab_,
                                                                       uint32_t tick) {
   using A = typename acc_of<T>::type;
-  __shared__ A plane[2][kQuadCells];
+  __shared__ __attribute__((aligned(16))) A plane[2][kQuadCells];
   __shared__ uint32_t tbl[32];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;

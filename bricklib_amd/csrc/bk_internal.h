@@ -41,8 +41,15 @@ struct PlanView {
   const uint32_t *mask;
   const uint32_t *flat;
   const uint32_t *start;
+  uint32_t nbricks;  // bricks of the adjacency the plan was built on
 };
 const PlanView plan_view(const bk_plan_s *p);
+
+// The quad schedule addresses its input through 32-bit element offsets:
+// brick * step + element must stay below 2^32.
+inline bool quad_offsets_fit(uint32_t nbricks, uint64_t step) {
+  return (uint64_t)nbricks * step <= 0x100000000ull;
+}
 
 // Per-brick kernels over L.list (any list).
 hipError_t launch_star(const SweepLaunch &L, hipStream_t s);

@@ -41,6 +41,7 @@ struct bk_plan_s {
   uint32_t *d_mask = nullptr;    // [nunits * kcol] member store mask per level
   uint32_t *d_flat = nullptr;    // [ncovered] covered bricks in plan order
   uint32_t *d_start = nullptr;   // [nunits] offset of each unit's bricks in d_flat
+  uint32_t nbricks = 0;          // bricks of the adjacency the plan was built on
   int device = 0;
 };
 
@@ -57,6 +58,7 @@ const PlanView plan_view(const bk_plan_s *p) {
   v.mask = p->d_mask;
   v.flat = p->d_flat;
   v.start = p->d_start;
+  v.nbricks = p->nbricks;
   return v;
 }
 
@@ -284,6 +286,7 @@ int bk_plan_create(const uint32_t *h_adj, uint32_t nbricks, const uint32_t *h_li
   p->kcol = kcol;
   p->nunits = nu;
   p->ncovered = nc;
+  p->nbricks = nbricks;
   hipGetDevice(&p->device);
   auto up = [](uint32_t **dst, const std::vector<uint32_t> &v) -> hipError_t {
     if (v.empty()) return hipSuccess;

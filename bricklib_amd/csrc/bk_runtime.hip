@@ -101,6 +101,13 @@ int check_desc(const bk_sweep_desc *d) {
     return fail(BK_ERR_ARG, "bk_sweep: input and output fields alias (use ping-pong fields)");
   if (d->in_step < 512 || d->out_step < 512)
     return fail(BK_ERR_ARG, "bk_sweep: step smaller than a brick (512 elements)");
+  if (d->plan && d->stencil == BK_CUBE125) {
+    const bk::PlanView P = bk::plan_view(d->plan);
+    if (P.shape == BK_PLAN_QUAD && !bk::quad_offsets_fit(P.nbricks, d->in_step))
+      return fail(BK_ERR_UNSUPPORTED,
+                  "bk_sweep: the quad schedule addresses at most 2^32 input elements "
+                  "(nbricks * step); use a smaller field or no plan");
+  }
   return BK_OK;
 }
 

@@ -184,7 +184,9 @@ int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n);
  * requires every listed brick's 27 neighbours to be where the quad's 4x4
  * column table puts them (true on periodic grids and on the interior of a
  * padded grid, not on the outer layer of a non-periodic one): otherwise
- * BK_ERR_UNSUPPORTED.  The
+ * BK_ERR_UNSUPPORTED.  A cube sweep through a quad plan addresses its input
+ * with 32-bit element offsets: nbricks * step must not exceed 2^32
+ * (BK_ERR_UNSUPPORTED from bk_sweep / bk_batch_set otherwise).  The
  * reference has no counterpart: its sweep is a loop over grid[k][j][i]
  * (stencils/3axis.cpp:51-66, weak/main.hip.cpp:274-279) and the plan is
  * how that loop is scheduled on the GPU.  Allocates device memory.
