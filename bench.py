@@ -325,15 +325,24 @@ def main():
                         "hbm_gbs": round(gb, 1), "hbm_frac": round(gb / HBM_PEAK_GBS, 4),
                         "tflops": round(tf, 2), "valu_frac": round(tf / peak_tf, 4)}
             print(f"[bench] {name}: {by[name]}", file=sys.stderr, flush=True)
-        # the same 7-point FP64 loop in three other forms: a separate refresh
+        # the same 7-point FP64 loop in other forms: a separate refresh
         # kernel before every sweep (the reference's order,
-        # weak/main.hip.cpp:249-279), the refresh once per 8 sweeps (sweeps in
-        # between over every brick, weak/main.cpp:26-36), and the
-        # wrapped-adjacency layout (no ghost bricks, no refresh)
+        # weak/main.hip.cpp:249-279), the same in the BrickDecomp layout
+        # (refresh = contiguous ghost runs from skin runs), the refresh once
+        # per 8 sweeps (sweeps in between over every brick,
+        # weak/main.cpp:26-36), and the wrapped-adjacency layout (no ghost
+        # bricks, no refresh)
         if not only or "7pt_f64_forms" in only:
             cur.clear()
             ms_deep = time_sweeps(bk, st7, a, b, g, stream, args.steps, 2, ghost_iters=8)
             ms_sep = time_sweeps(bk, st7, a, b, g, stream, args.steps, 2)
+            from bricklib_amd.decomp import DecompGrid
+            gd = DecompGrid(nb, stream=stream)
+            da = bk.Brick(gd.info, gd.field_storage(bk.BK_F64), 0)
+            db = bk.Brick(gd.info, gd.field_storage(bk.BK_F64), 0)
+            gd.fill_uniform(da, 1234, stream=stream)
+            ms_decomp = time_sweeps(bk, st7, da, db, gd, stream, args.steps, 2)
+            del da, db, gd
             gw = bk.PeriodicGrid(nb, stream=stream, ghost=0)
             wa = bk.Brick(gw.info, gw.field_storage(bk.BK_F64), 0)
             wb = bk.Brick(gw.info, gw.field_storage(bk.BK_F64), 0)
@@ -341,6 +350,7 @@ def main():
             ms_wrap = time_sweeps(bk, st7, wa, wb, gw, stream, args.steps, 2)
             del wa, wb, gw
             for name, ms_s in (("7pt_f64_separate_refresh", ms_sep),
+                               ("7pt_f64_brickdecomp_layout", ms_decomp),
                                ("7pt_f64_refresh_every_8", ms_deep),
                                ("7pt_f64_wrapped_no_ghost", ms_wrap)):
                 by[name] = {"gstencil_s": round(pts / (ms_s * 1e-3) / 1e9, 2),

@@ -46,6 +46,19 @@ class bk_sweep_desc(C.Structure):
     ]
 
 
+class bk_decomp_region(C.Structure):
+    _fields_ = [
+        ("neighbor", C.c_int32 * 3),
+        ("ghost_pos", C.c_uint32),
+        ("skin_pos", C.c_uint32),
+        ("len", C.c_uint32),
+        ("skin_st", C.c_uint32),
+        ("skin_ed", C.c_uint32),
+        ("first_pad", C.c_uint32),
+        ("last_pad", C.c_uint32),
+    ]
+
+
 class bk_batch_req(C.Structure):
     _fields_ = [
         ("stencil", C.c_int),
@@ -106,6 +119,9 @@ SIGNATURES = {
     "bk_sweeps_fused": (C.c_int, [C.POINTER(bk_sweep_desc), vp, vp, vp, C.c_uint32, C.c_int,
                                   C.c_int, vp]),
     "bk_ghost_images": (C.c_int, [longp, u32p, u32p, C.POINTER(C.c_uint32)]),
+    "bk_decomp_layout": (C.c_int, [longp, C.c_int, C.c_size_t, C.c_size_t, u32p, u32p,
+                                   C.POINTER(C.c_uint32), u32p, C.POINTER(bk_decomp_region),
+                                   C.POINTER(C.c_uint32)]),
     "bk_graph_cache_clear": (C.c_int, []),
     "bk_fill_uniform": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_uint64, C.c_double,
                                   C.c_double, vp]),

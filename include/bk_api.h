@@ -167,6 +167,36 @@ int bk_ghost_pairs(const long nb[3], uint32_t *h_pairs, uint32_t *npairs);
  * query *nimg.  Feeds the fused refresh of bk_sweeps_fused. */
 int bk_ghost_images(const long nb[3], uint32_t *h_off, uint32_t *h_img,
                     uint32_t *nimg);
+/* The BrickDecomp layout (BrickDecomp::initialize, brick-mpi.h:322-460,
+ * with the skin3d_good surface order, src/brick-mpi.cpp:25-52) of one
+ * subdomain of nb bricks per axis (i, j, k) with a ghost zone of
+ * ghost_bricks bricks: ids grouped as inner bricks, skin, ghost, each
+ * neighbour's ghost bricks contiguous.  Bricks of brick_bytes are grouped
+ * to whole pages of page_bytes (ids start at page/brick when that exceeds 1,
+ * regions padded to it), as the reference does for mmap.  Outputs:
+ *   h_grid   (nb+2g)^3 ids, i fastest (the grid array bDecomp[k][j][i]);
+ *   h_adj    nbricks x 27 adjacency, out-of-range neighbours = brick 0;
+ *   sep_pos  first skin id, first ghost id, end of ghost ids;
+ *   h_regions one entry per contiguous exchange run: the ghost bricks
+ *            [ghost_pos, +len) owned by neighbour `neighbor` and the skin
+ *            bricks [skin_pos, +len) the opposite neighbour receives; on a
+ *            periodic single subdomain the first run is a copy of the second.
+ * Any output array may be NULL; call with NULLs to size them (*nbricks,
+ * *nregions). */
+typedef struct bk_decomp_region {
+  int32_t neighbor[3];  /* -1, 0, +1 per axis (i, j, k) */
+  uint32_t ghost_pos;
+  uint32_t skin_pos;
+  uint32_t len;
+  uint32_t skin_st;     /* skin-list positions [skin_st, skin_ed) */
+  uint32_t skin_ed;
+  uint32_t first_pad;   /* padding bricks at the run's start / end */
+  uint32_t last_pad;
+} bk_decomp_region;
+int bk_decomp_layout(const long nb[3], int ghost_bricks, size_t brick_bytes,
+                     size_t page_bytes, uint32_t *h_grid, uint32_t *h_adj,
+                     uint32_t *nbricks, uint32_t sep_pos[3],
+                     bk_decomp_region *h_regions, uint32_t *nregions);
 /* Interior brick list in grid order (k slowest, i fastest) for the same
  * padded grid: the bricks the sweep loops of stencils/3axis.cpp:51-66 visit.
  * h_list may be NULL to query *n. */
