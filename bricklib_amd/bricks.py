@@ -429,11 +429,48 @@ class PeriodicGrid:
         columns of 8 bricks for the radius-4 star, and one brick per wave
         (no plan) for the radius-1 and radius-2 stars."""
         from ._lib import BK_CUBE125, BK_STAR25
+        tuned = getattr(self, "_tuned", {}).get(int(stencil_kind))
+        if tuned is not None:
+            return None if tuned == "none" else self.plan(*tuned)
         if stencil_kind == BK_CUBE125:
             return self.plan("quad", 8)
         if stencil_kind == BK_STAR25:
             return self.plan("column", 8)
         return None
+
+    def tune(self, st, dtype: int = BK_F64, steps: int = 5, stream=None,
+             candidates=None) -> dict:
+        """Time every candidate schedule of stencil `st` on this grid (each
+        `steps` sweeps without refresh, after one warm-up) and make the
+        fastest this grid's choice for plan="auto".  All schedules compute
+        the same bits, so the choice only changes the speed.  Candidates:
+        no plan and column plans of 4, 8 and 16 bricks for the stars; no
+        plan and quad plans of 4, 8 and 16 levels for the cube.  Returns
+        {"best": ..., "ms": {candidate: ms per sweep}}."""
+        from ._lib import BK_CUBE125
+        if candidates is None:
+            shape = "quad" if st.kind == BK_CUBE125 else "column"
+            candidates = ["none"] + [(shape, k) for k in (4, 8, 16)]
+        a = Brick(self.info, self.field_storage(dtype), 0)
+        b = Brick(self.info, self.field_storage(dtype), 0)
+        self.fill_uniform(a, 99, stream=stream)
+        s = stream if stream is not None else Stream()
+        times = {}
+        for cand in candidates:
+            pl = None if cand == "none" else self.plan(*cand)
+            sweeps(st, a, b, self, 1, refresh=False, stream=s, plan=pl)
+            s.sync()
+            e0, e1 = Event(), Event()
+            e0.record(s)
+            sweeps(st, a, b, self, steps, refresh=False, stream=s, plan=pl)
+            e1.record(s)
+            e1.sync()
+            times[cand] = e0.elapsed_ms(e1) / steps
+        best = min(times, key=times.get)
+        if not hasattr(self, "_tuned"):
+            self._tuned = {}
+        self._tuned[int(st.kind)] = best
+        return {"best": best, "ms": {str(k): round(v, 4) for k, v in times.items()}}
 
     @property
     def nbricks(self) -> int:

@@ -356,3 +356,21 @@ def test_fused_refresh_rejects_plans_and_cube(gpu):
     with pytest.raises(BrickError):
         gpu.sweeps(S.Stencil.make("25pt", seed=1), a, b, g, 2, fused=True,
                    plan=g.plan("column", 8))
+
+
+@pytest.mark.parametrize("name", ["13pt", "125pt"])
+def test_tuner_picks_a_schedule_with_the_same_bits(gpu, name):
+    g = gpu.PeriodicGrid((4, 4, 12))
+    st = S.Stencil.make(name, seed=8)
+    a = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    b = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    g.fill_uniform(a, 2)
+    before = gpu.sweeps(st, a, b, g, 3).storage.to_host()[g.interior]
+    out = g.tune(st, BK_F64, steps=2)
+    assert set(out["ms"]) == {"none", f"('{'quad' if name == '125pt' else 'column'}', 4)",
+                              f"('{'quad' if name == '125pt' else 'column'}', 8)",
+                              f"('{'quad' if name == '125pt' else 'column'}', 16)"}
+    assert str(out["best"]) in out["ms"]
+    g.fill_uniform(a, 2)
+    after = gpu.sweeps(st, a, b, g, 3).storage.to_host()[g.interior]
+    np.testing.assert_array_equal(after, before)
