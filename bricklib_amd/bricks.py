@@ -439,14 +439,19 @@ class PeriodicGrid:
         return None
 
     def tune(self, st, dtype: int = BK_F64, steps: int = 5, stream=None,
-             candidates=None) -> dict:
-        """Time every candidate schedule of stencil `st` on this grid (each
-        `steps` sweeps without refresh, after one warm-up) and make the
-        fastest this grid's choice for plan="auto".  All schedules compute
-        the same bits, so the choice only changes the speed.  Candidates:
-        no plan and column plans of 4, 8 and 16 bricks for the stars; no
-        plan and quad plans of 4, 8 and 16 levels for the cube.  Returns
-        {"best": ..., "ms": {candidate: ms per sweep}}."""
+             candidates=None, rounds: int = 3, warm_s: float = 0.3) -> dict:
+        """Time every candidate schedule of stencil `st` on this grid and
+        make the fastest this grid's choice for plan="auto".  All schedules
+        compute the same bits, so the choice only changes the speed.
+        Candidates: no plan and column plans of 4, 8 and 16 bricks for the
+        stars; no plan and quad plans of 4, 8 and 16 levels for the cube.
+        The candidates first run untimed for warm_s seconds (the GPU's clock
+        rises over the first fraction of a second of sustained work, which
+        would favour whichever candidate is timed last), then each is timed
+        over `steps` sweeps without refresh in `rounds` interleaved rounds,
+        keeping its best.  Returns {"best": ..., "ms": {candidate: ms per
+        sweep}}."""
+        import time
         from ._lib import BK_CUBE125
         if candidates is None:
             shape = "quad" if st.kind == BK_CUBE125 else "column"
@@ -455,17 +460,24 @@ class PeriodicGrid:
         b = Brick(self.info, self.field_storage(dtype), 0)
         self.fill_uniform(a, 99, stream=stream)
         s = stream if stream is not None else Stream()
-        times = {}
-        for cand in candidates:
-            pl = None if cand == "none" else self.plan(*cand)
+        plans = {c: None if c == "none" else self.plan(*c) for c in candidates}
+        for pl in plans.values():
             sweeps(st, a, b, self, 1, refresh=False, stream=s, plan=pl)
+        s.sync()
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < warm_s:
+            for pl in plans.values():
+                sweeps(st, a, b, self, steps, refresh=False, stream=s, plan=pl)
             s.sync()
-            e0, e1 = Event(), Event()
-            e0.record(s)
-            sweeps(st, a, b, self, steps, refresh=False, stream=s, plan=pl)
-            e1.record(s)
-            e1.sync()
-            times[cand] = e0.elapsed_ms(e1) / steps
+        times = {c: float("inf") for c in candidates}
+        e0, e1 = Event(), Event()
+        for _ in range(max(1, rounds)):
+            for cand, pl in plans.items():
+                e0.record(s)
+                sweeps(st, a, b, self, steps, refresh=False, stream=s, plan=pl)
+                e1.record(s)
+                e1.sync()
+                times[cand] = min(times[cand], e0.elapsed_ms(e1) / steps)
         best = min(times, key=times.get)
         if not hasattr(self, "_tuned"):
             self._tuned = {}

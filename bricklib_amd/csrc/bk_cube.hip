@@ -283,6 +283,34 @@ __device__ __forceinline__ void quad_plane_steady(const A *pl, int lj, int li,
   quad_steady_group<0, A>(pl, xo, cp, cf, x, acc);
 }
 
+// The other members of the quad anchored at m0 (m1 = i+1, m2 = j+1,
+// m3 = i+1,j+1) and the brick tables of the 4x4 block of columns around it:
+// level 0 in tbl[16..32), its k-1 neighbours (the input level of the first
+// two planes) in tbl[0..16).  Entry (r, c) comes from one member's in-plane
+// neighbours: m0 for r, c < 3, m1 for column 3, m2 for row 3, m3 for the
+// corner -- one writer per entry.  Level 0 is taken from the in-plane
+// neighbours directly, not by walking up from level -1 (where level -1 lies
+// outside the grid its adjacency entries stand for no brick).  The plan
+// builder has checked that every stored member's 27 neighbours are what
+// these tables give.
+__device__ __forceinline__ void quad_tables(const uint32_t *__restrict__ adj, uint32_t m0,
+                                            uint32_t &m1, uint32_t &m2, uint32_t &m3,
+                                            uint32_t *tbl, int lane) {
+  m1 = adj[(size_t)m0 * 27 + 14];
+  m2 = adj[(size_t)m0 * 27 + 16];
+  m3 = adj[(size_t)m1 * 27 + 16];
+  if (lane < 16) {
+    const int r = lane >> 2, c = lane & 3;
+    const uint32_t m = r < 3 ? (c < 3 ? m0 : m1) : (c < 3 ? m2 : m3);
+    const int jd = r < 3 ? r : 2, id = c < 3 ? c : 2;
+    const int jj = (r == 3) ? 2 : jd, ii = (c == 3) ? 2 : id;
+    const uint32_t n0 = adj[(size_t)m * 27 + 9 + 3 * jj + ii];
+    tbl[lane] = adj[(size_t)n0 * 27 + 4];
+    tbl[16 + lane] = n0;
+  }
+  __syncthreads();
+}
+
 // One plan unit of the quad schedule: members m0.. over L levels, store
 // masks mk[0..L), coefficients through the constant-address pointer cp.
 // plane: two LDS plane buffers, tbl: 32-entry LDS brick table (the current
@@ -297,31 +325,10 @@ __device__ __forceinline__ void cube_quad_unit(
   using A = typename acc_of<T>::type;
   // the padded plane in 2-cell chunks: a 20-cell row is 10 chunks, each
   // inside one brick row and 2-cell aligned there (16 bytes in FP64)
-  constexpr int PW = kQuadPW, CPR = kQuadPW / 2, NCH = kQuadPW * CPR, NLD = (NCH + 63) / 64;
+  constexpr int CPR = kQuadPW / 2, NCH = kQuadPW * CPR, NLD = (NCH + 63) / 64;
   const int lj = lane >> 3, li = lane & 7;
-  const uint32_t m1 = adj[(size_t)m0 * 27 + 14];
-  const uint32_t m2 = adj[(size_t)m0 * 27 + 16];
-  const uint32_t m3 = adj[(size_t)m1 * 27 + 16];
-
-  // brick tables of the 4x4 block of columns around the quad: level 0 in
-  // tbl[16..32), its k-1 neighbours (the input level of the first two
-  // planes) in tbl[0..16).  Entry (r, c) comes from one member's in-plane
-  // neighbours: m0 for r, c < 3, m1 for column 3, m2 for row 3, m3 for the
-  // corner -- one writer per entry.  Level 0 is taken from the in-plane
-  // neighbours directly, not by walking up from level -1 (where level -1
-  // lies outside the grid its adjacency entries stand for no brick).  The
-  // plan builder has checked that every stored member's 27 neighbours are
-  // what these tables give.
-  if (lane < 16) {
-    const int r = lane >> 2, c = lane & 3;
-    const uint32_t m = r < 3 ? (c < 3 ? m0 : m1) : (c < 3 ? m2 : m3);
-    const int jd = r < 3 ? r : 2, id = c < 3 ? c : 2;
-    const int jj = (r == 3) ? 2 : jd, ii = (c == 3) ? 2 : id;
-    const uint32_t n0 = adj[(size_t)m * 27 + 9 + 3 * jj + ii];
-    tbl[lane] = adj[(size_t)n0 * 27 + 4];
-    tbl[16 + lane] = n0;
-  }
-  __syncthreads();
+  uint32_t m1, m2, m3;
+  quad_tables(adj, m0, m1, m2, m3, tbl, lane);
 
   // Load t of a plane fetches chunk q = 64t + lane of the 20x20 padded
   // plane, two cells in one load (the last load only for the first
@@ -445,6 +452,170 @@ __device__ __forceinline__ void cube_quad_unit(
   }
 }
 
+// FP64 form of the quad unit.  The plane moves from the input field into
+// LDS by direct loads (global_load_lds_dwordx4): load t, lane l fetches
+// chunk q = 64t + l of the plane taken as rows of 12 two-cell chunks (10 of
+// data, 2 of padding), and the hardware writes it at byte 16q of the LDS
+// plane -- the 24-cell row stride of the padded plane, with no register
+// round trip.  Padding chunks and chunks past row 19 fetch a cell of the
+// plane again and land in cells no read touches.  Each plane's 6x6 window
+// of the lane's 2x2 points is read into registers once; the single LDS
+// plane then takes the next plane's loads while the window computes, slot
+// by slot (the 25 coefficients of one output plane per scalar wait).
+constexpr int kQuadDmaCPR = 12, kQuadDmaLoads = 4;
+static_assert(kQuadDmaLoads * 64 * 2 <= 2 * kQuadCells, "the direct-load plane fits");
+static_assert(kQuadDmaLoads * 64 >= kQuadPW * kQuadDmaCPR, "four loads cover the plane");
+static_assert(kQuadDmaCPR * 2 == kQuadStride, "chunk rows are LDS rows");
+typedef __attribute__((address_space(3))) void *lds_void_ptr;
+
+// 16 bytes per lane from global memory straight into LDS: lane l's bytes land
+// at lds + 16 l (lds is wave-uniform)
+__device__ __forceinline__ void load_lds16(const void *g, void *lds) {
+  __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds, 16, 0, 0);
+}
+
+// Slot s of input plane P (output plane P - 2 + s, tap plane dk = 2 - s):
+// per output point the taps go (dj, di) lexicographic within the slot, and
+// successive planes bring dk = -2 .. 2, so the order per point is the
+// per-brick kernel's.
+template <class A>
+__device__ __forceinline__ void quad_plane_window(const A (&x)[6][6], kconst_ptr<A> cp, int P,
+                                                  int nout, A (&acc)[5][2][2]) {
+#pragma unroll
+  for (int s = 0; s < 5; ++s) {
+    const int O = P - 2 + s;
+    if (O < 0 || O >= nout) continue;
+    asm volatile("" : "+s"(cp));
+    const int dk = 2 - s;
+#pragma unroll
+    for (int dj = -2; dj <= 2; ++dj)
+#pragma unroll
+      for (int di = -2; di <= 2; ++di) {
+        const A cf = cp[(dk + 2) * 25 + (dj + 2) * 5 + (di + 2)];
+#pragma unroll
+        for (int pj = 0; pj < 2; ++pj)
+#pragma unroll
+          for (int pi = 0; pi < 2; ++pi) {
+            const A xv = x[pj + dj + 2][pi + di + 2];
+            if (s == 4 && dj == -2 && di == -2) acc[s][pj][pi] = cf * xv;
+            else acc[s][pj][pi] = fmadd(cf, xv, acc[s][pj][pi]);
+          }
+      }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void cube_quad_unit_dma(
+    const uint32_t *__restrict__ adj, const T *__restrict__ in, T *__restrict__ out, size_t st,
+    size_t ost, uint32_t m0, int L, const uint32_t *__restrict__ mk,
+    kconst_ptr<typename acc_of<T>::type> cp0, typename acc_of<T>::type *lds, uint32_t *tbl,
+    int lane) {
+  using A = typename acc_of<T>::type;
+  static_assert(sizeof(T) == 8 && sizeof(A) == 8, "a 16-byte direct load carries two cells");
+  constexpr int CPR = kQuadDmaCPR, NLD = kQuadDmaLoads;
+  const int lj = lane >> 3, li = lane & 7;
+  uint32_t m1, m2, m3;
+  quad_tables(adj, m0, m1, m2, m3, tbl, lane);
+
+  // one 32-bit element offset per load, recomputed at each new input level
+  // (fields beyond 2^32 elements are rejected for the quad schedule)
+  const uint32_t st32 = (uint32_t)st;
+  uint32_t off[NLD];
+  auto set_offsets = [&]() {
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+#pragma unroll
+    for (int t = 0; t < NLD; ++t) {
+      const int q = t * 64 + ln;
+      const int row = min(q / CPR, kQuadPW - 1), cc = min(q % CPR, kQuadPW / 2 - 1);
+      const int J = row + 6, I = cc * 2 + 6;  // cell + 8 in the 4x4 block
+      off[t] = tbl[(J >> 3) * 4 + (I >> 3)] * st32 + (uint32_t)((J & 7) * 8 + (I & 7));
+    }
+  };
+  auto load_plane = [&](uint32_t kofs) {
+#pragma unroll
+    for (int t = 0; t < NLD; ++t)
+      load_lds16(in + off[t] + kofs, lds + t * 128);
+  };
+  set_offsets();
+  uint32_t tn = 0;  // next level's table entry (lanes 0..15)
+  if (lane < 16) tn = tbl[16 + lane];
+  load_plane(6 * 64);
+
+  const int mm_l = (lj >> 2) * 2 + (li >> 2);
+  const int jo = (2 * lj) & 7, io = (2 * li) & 7;
+  const int xo = (2 * lj) * kQuadStride + 2 * li;
+  uint32_t om0 = m0, om1 = m1, om2 = m2, om3 = m3;
+
+  A acc[5][2][2];
+#pragma unroll
+  for (int s = 0; s < 5; ++s)
+#pragma unroll
+    for (int pj = 0; pj < 2; ++pj)
+#pragma unroll
+      for (int pi = 0; pi < 2; ++pi) acc[s][pj][pi] = A(0);
+
+  const int nplanes = 8 * L + 4;
+  const int nout = 8 * L;
+  for (int it = 0; it < nplanes; ++it) {
+    const int P = it - 2;
+    // the plane's direct loads have landed in LDS
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    A x[6][6];
+    typedef A A2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; c += 2) {
+        const A2 v = *reinterpret_cast<const A2 *>(&lds[xo + r * kQuadStride + c]);
+        x[r][c] = v.x;
+        x[r][c + 1] = v.y;
+      }
+    // the window is in registers before the next plane overwrites LDS
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    if (it + 1 < nplanes) {
+      const int Pn = P + 1;
+      if ((Pn & 7) == 0) {  // next plane starts a new input level
+        if (lane < 16) tbl[lane] = tn;
+        __syncthreads();
+        set_offsets();
+        if (lane < 16) tn = adj[(size_t)tbl[lane] * 27 + 22];
+      }
+      load_plane((uint32_t)((Pn & 7) * 64));
+    }
+
+    quad_plane_window<A>(x, cp0, P, nout, acc);
+
+    const int O = P - 2;  // slot 0 is complete
+    if (O >= 0 && O < nout) {
+      const int lo = O >> 3, kk = O & 7;
+      if (kk == 0 && lo > 0) {
+        om0 = adj[(size_t)om0 * 27 + 22];
+        om1 = adj[(size_t)om1 * 27 + 22];
+        om2 = adj[(size_t)om2 * 27 + 22];
+        om3 = adj[(size_t)om3 * 27 + 22];
+      }
+      if ((mk[lo] >> mm_l) & 1u) {
+        const uint32_t ob = mm_l == 0 ? om0 : (mm_l == 1 ? om1 : (mm_l == 2 ? om2 : om3));
+        T *po = out + (size_t)ob * ost + kk * 64 + jo * 8 + io;
+        narrow(po[0], acc[0][0][0]);
+        narrow(po[1], acc[0][0][1]);
+        narrow(po[8], acc[0][1][0]);
+        narrow(po[9], acc[0][1][1]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int pj = 0; pj < 2; ++pj)
+#pragma unroll
+        for (int pi = 0; pi < 2; ++pi) acc[s][pj][pi] = acc[s + 1][pj][pi];
+  }
+}
+
+// FP64: 3 waves per SIMD (135 VGPRs); capping at 128 for a fourth wave
+// spills and runs slower
 template <class T>
 __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
   using A = typename acc_of<T>::type;
@@ -458,14 +629,18 @@ __global__ __launch_bounds__(64) void cube_sweep_quads(const CubeArgs<T> a) {
   const kconst_ptr<A> cp =
       (kconst_ptr<A>)((kconst_ptr<char>)__builtin_amdgcn_kernarg_segment_ptr() +
                       offsetof(CubeArgs<T>, c));
-  cube_quad_unit<T>(a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], (int)a.len[u],
-                    a.mask + (size_t)u * a.kcol, cp, plane, tbl, threadIdx.x);
+  if constexpr (sizeof(T) == 8)
+    cube_quad_unit_dma<T>(a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], (int)a.len[u],
+                          a.mask + (size_t)u * a.kcol, cp, &plane[0][0], tbl, threadIdx.x);
+  else
+    cube_quad_unit<T>(a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], (int)a.len[u],
+                      a.mask + (size_t)u * a.kcol, cp, plane, tbl, threadIdx.x);
 }
 
 // Batched variant: one persistent launch serves every cube request of a
 // group table; work item = one quad-plan unit of one request.
 template <class T>
-__global__ __launch_bounds__(64, sizeof(T) == 8 ? 4 : 1) void cube_sweep_batched(const BatchTable *tab_,
+__global__ __launch_bounds__(64) void cube_sweep_batched(const BatchTable *tab_,
                                                                       uint32_t tick) {
   using A = typename acc_of<T>::type;
   __shared__ __attribute__((aligned(16))) A plane[2][kQuadCells];
@@ -482,9 +657,14 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 4 : 1) void cube_sweep_batched
     const bool odd = ((sg.parity + tick) & 1u) != 0;
     const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
     T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);
-    cube_quad_unit<T>(sg.adj, in, out, sg.step, sg.step, sg.items[ul], (int)sg.len[ul],
-                      sg.mask + (size_t)ul * sg.kcol, (kconst_ptr<A>)sg.coeff, plane, tbl,
-                      threadIdx.x);
+    if constexpr (sizeof(T) == 8)
+      cube_quad_unit_dma<T>(sg.adj, in, out, sg.step, sg.step, sg.items[ul], (int)sg.len[ul],
+                            sg.mask + (size_t)ul * sg.kcol, (kconst_ptr<A>)sg.coeff,
+                            &plane[0][0], tbl, threadIdx.x);
+    else
+      cube_quad_unit<T>(sg.adj, in, out, sg.step, sg.step, sg.items[ul], (int)sg.len[ul],
+                        sg.mask + (size_t)ul * sg.kcol, (kconst_ptr<A>)sg.coeff, plane, tbl,
+                        threadIdx.x);
   }
 }
 

@@ -366,7 +366,7 @@ def test_tuner_picks_a_schedule_with_the_same_bits(gpu, name):
     b = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
     g.fill_uniform(a, 2)
     before = gpu.sweeps(st, a, b, g, 3).storage.to_host()[g.interior]
-    out = g.tune(st, BK_F64, steps=2)
+    out = g.tune(st, BK_F64, steps=2, warm_s=0.05)
     assert set(out["ms"]) == {"none", f"('{'quad' if name == '125pt' else 'column'}', 4)",
                               f"('{'quad' if name == '125pt' else 'column'}', 8)",
                               f"('{'quad' if name == '125pt' else 'column'}', 16)"}
