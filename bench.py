@@ -140,6 +140,44 @@ def copy_peak(bk, stream, nbytes=2 << 30, reps=20):
     return max(out.values()), out
 
 
+def valu_peak(bk, stream, warm_s=0.3):
+    """Sustained VALU rate of the part (TFLOP/s, FP64 and FP32 without
+    packed instructions): the best of the library's register-only
+    multiply-add probe over 1 and 8 independent chains per lane at 4 and 8
+    waves per SIMD, after warm_s seconds of untimed probe launches."""
+    from bricklib_amd import DeviceBuffer, Event
+    from bricklib_amd import _lib
+    import ctypes as C
+    lib = bk.load()
+    n = 8 * 1024 * 4 * 64  # up to 8 waves on each SIMD of up to 1024 CUs
+    out = DeviceBuffer(n * 8)
+    fl = C.c_double(0)
+    best = {}
+    for dt, name in ((bk.BK_F64, "f64"), (bk.BK_F32, "f32")):
+        def launch(chains, waves, iters):
+            _lib.check(lib.bk_valu_probe(dt, chains, waves, iters, C.c_void_p(out.ptr), n,
+                                         C.byref(fl), C.c_void_p(int(stream))))
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < warm_s:
+            launch(8, 8, 2000)
+            stream.sync()
+        rates = {}
+        for chains in (1, 8):
+            for waves in (4, 8):
+                iters = 8000 // waves
+                launch(chains, waves, iters)
+                e0, e1 = Event(), Event()
+                e0.record(stream)
+                for _ in range(3):
+                    launch(chains, waves, iters)
+                e1.record(stream)
+                e1.sync()
+                rates[f"chains{chains}_waves{waves}"] = round(
+                    3 * fl.value / (e0.elapsed_ms(e1) * 1e-3) / 1e12, 2)
+        best[name] = {"tflops": max(rates.values()), "runs": rates}
+    return best
+
+
 def cpu_baseline(nb, coeff, seconds):
     """Oracle brick loop (reference order, 3axis.cpp:51-66) on the host."""
     import oracle as O
@@ -310,7 +348,10 @@ def main():
 
     by = {}
     cur = {}
+    vpeak = {}
     if not args.no_extra:
+        vpeak = valu_peak(bk, stream)
+        print(f"[bench] measured VALU peak: {vpeak}", file=sys.stderr, flush=True)
         only = set(x for x in args.only.split(",") if x)
         cases = [("13pt_f64", "13pt", bk.BK_F64), ("25pt_f64", "25pt", bk.BK_F64),
                  ("125pt_f64", "125pt", bk.BK_F64), ("cond_f64", "cond", bk.BK_F64),
@@ -338,6 +379,10 @@ def main():
                         "ms_per_step": round(ms_s, 4), "kernel_ms": round(ms_k, 4),
                         "hbm_gbs": round(gb, 1), "hbm_frac": round(gb / HBM_PEAK_GBS, 4),
                         "tflops": round(tf, 2), "valu_frac": round(tf / peak_tf, 4)}
+            vm = vpeak.get("f64" if dt == bk.BK_F64 else "f32", {}).get("tflops")
+            if sname == "125pt" and vm:
+                # the measured peak is the non-packed rate for both dtypes
+                by[name]["valu_frac_of_measured"] = round(tf / vm, 4)
             print(f"[bench] {name}: {by[name]}", file=sys.stderr, flush=True)
         # the same 7-point FP64 loop in other forms: a separate refresh
         # kernel before every sweep (the reference's order,
@@ -430,6 +475,7 @@ def main():
                      "measured_copy_gbs": round(peak_copy, 1), "copy_kernels_gbs": copies,
                      "frac_of_measured_copy": round(achieved / peak_copy, 4)},
         "cpu_baseline": cpu,
+        "valu_peak_measured": vpeak,
         "by_stencil": by,
         "precision": prec,
         "serving": serving,

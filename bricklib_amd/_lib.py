@@ -100,6 +100,8 @@ SIGNATURES = {
     "bk_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "bk_zero": (C.c_int, [vp, C.c_size_t, vp]),
     "bk_stream_copy": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "bk_valu_probe": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint32, vp, C.c_size_t,
+                                C.POINTER(C.c_double), vp]),
     "bk_stream_create": (C.c_int, [C.POINTER(vp)]),
     "bk_stream_destroy": (C.c_int, [vp]),
     "bk_stream_sync": (C.c_int, [vp]),

@@ -424,16 +424,18 @@ class PeriodicGrid:
         return self._d_every
 
     def auto_plan(self, stencil_kind: int):
-        """Default schedule per stencil, from measurements at 512^3 on
-        MI355X: 2x2 column quads of 8 levels for the cube, k-marching
-        columns of 8 bricks for the radius-4 star, and one brick per wave
-        (no plan) for the radius-1 and radius-2 stars."""
+        """Default schedule per stencil, from warmed measurements at 512^3
+        on MI355X: 2x2 column quads of 4 levels for the cube (1-3% ahead
+        of 8 levels, whose smaller halo share is outweighed by the shorter
+        tail of more, shorter units), k-marching columns of 8 bricks for
+        the radius-4 star, and one brick per wave (no plan) for the
+        radius-1 and radius-2 stars."""
         from ._lib import BK_CUBE125, BK_STAR25
         tuned = getattr(self, "_tuned", {}).get(int(stencil_kind))
         if tuned is not None:
             return None if tuned == "none" else self.plan(*tuned)
         if stencil_kind == BK_CUBE125:
-            return self.plan("quad", 8)
+            return self.plan("quad", 4)
         if stencil_kind == BK_STAR25:
             return self.plan("column", 8)
         return None

@@ -80,6 +80,36 @@ __global__ __launch_bounds__(256) void stream_copy16(const u32x4 *__restrict__ s
   for (; q < n; q += 256) d[q] = s[q];
 }
 
+// ---------------------------------------------------------------------------
+// register-only multiply-add loop used to measure the VALU rate the part
+// sustains: C independent chains per lane, the coefficient wave-uniform (an
+// SGPR operand, as in the stencil kernels), 64 FMAs per iteration
+// ---------------------------------------------------------------------------
+template <class T, int C>
+__global__ __launch_bounds__(64) void valu_fma_probe(T *__restrict__ out, T c0, T c1,
+                                                     uint32_t iters) {
+  T acc[C], x[C];
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    acc[i] = T(threadIdx.x) * T(1e-3) + T(i);
+    x[i] = T(threadIdx.x + i + 1) * T(0.5);
+  }
+  for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int r = 0; r < 32 / C; ++r)
+#pragma unroll
+      for (int i = 0; i < C; ++i) acc[i] = fmadd(c0, x[i], acc[i]);
+#pragma unroll
+    for (int r = 0; r < 32 / C; ++r)
+#pragma unroll
+      for (int i = 0; i < C; ++i) acc[i] = fmadd(c1, x[i], acc[i]);
+  }
+  T s = T(0);
+#pragma unroll
+  for (int i = 0; i < C; ++i) s += acc[i];
+  out[(size_t)blockIdx.x * 64 + threadIdx.x] = s;
+}
+
 // Batched periodic refresh: one persistent launch refreshes the input field
 // of every request of a table (all of one dtype); work item = one (dst,
 // src) ghost pair of one request; NV 16-byte vectors per lane per brick.
@@ -364,6 +394,26 @@ hipError_t launch_stream_copy(void *dst, const void *src, size_t bytes, hipStrea
   if (nblk > cap) nblk = cap;
   hipLaunchKernelGGL(stream_copy16, dim3((uint32_t)nblk), dim3(256), 0, s,
                      static_cast<const u32x4 *>(src), static_cast<u32x4 *>(dst), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_valu_probe(int dtype, int chains, uint32_t nblocks, uint32_t iters, void *out,
+                             hipStream_t s) {
+  const dim3 g(nblocks), b(64);
+  if (dtype == BK_F64 && chains == 1)
+    hipLaunchKernelGGL((valu_fma_probe<double, 1>), g, b, 0, s, static_cast<double *>(out),
+                       0.999, 1.0001, iters);
+  else if (dtype == BK_F64 && chains == 8)
+    hipLaunchKernelGGL((valu_fma_probe<double, 8>), g, b, 0, s, static_cast<double *>(out),
+                       0.999, 1.0001, iters);
+  else if (dtype == BK_F32 && chains == 1)
+    hipLaunchKernelGGL((valu_fma_probe<float, 1>), g, b, 0, s, static_cast<float *>(out),
+                       0.999f, 1.0001f, iters);
+  else if (dtype == BK_F32 && chains == 8)
+    hipLaunchKernelGGL((valu_fma_probe<float, 8>), g, b, 0, s, static_cast<float *>(out),
+                       0.999f, 1.0001f, iters);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

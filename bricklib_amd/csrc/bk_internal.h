@@ -103,6 +103,8 @@ hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,
                                 uint32_t npairs, hipStream_t s);
 hipError_t launch_zero(void *dst, size_t bytes, hipStream_t s);
 hipError_t launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+hipError_t launch_valu_probe(int dtype, int chains, uint32_t nblocks, uint32_t iters, void *out,
+                             hipStream_t s);
 hipError_t launch_fill_uniform(int dtype, void *dat, size_t step,
                                const long nb[3], int ghost, uint64_t seed,
                                double lo, double hi, hipStream_t s);

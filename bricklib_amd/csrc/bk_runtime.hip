@@ -364,6 +364,29 @@ int bk_stream_copy(void *d_dst, const void *d_src, size_t bytes, bk_stream s) {
   return BK_OK;
 }
 
+int bk_valu_probe(int dtype, int chains, int waves_per_simd, uint32_t iters, void *d_out,
+                  size_t out_elems, double *flops, bk_stream s) {
+  if (dtype != BK_F64 && dtype != BK_F32)
+    return fail(BK_ERR_ARG, "bk_valu_probe: dtype must be BK_F64 or BK_F32");
+  if (chains != 1 && chains != 8) return fail(BK_ERR_ARG, "bk_valu_probe: chains must be 1 or 8");
+  if (waves_per_simd < 1 || waves_per_simd > 8)
+    return fail(BK_ERR_ARG, "bk_valu_probe: waves_per_simd in [1, 8]");
+  if (!d_out || !flops) return fail(BK_ERR_ARG, "bk_valu_probe: null pointer");
+  int dev = 0, cus = 0;
+  BK_HIP(hipGetDevice(&dev));
+  BK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const uint64_t nblocks = (uint64_t)waves_per_simd * (uint64_t)cus * 4u;  // 4 SIMDs per CU
+  if (out_elems < nblocks * 64) {
+    char msg[160];
+    snprintf(msg, sizeof msg, "bk_valu_probe: d_out holds %zu elements, the launch stores %llu",
+             out_elems, (unsigned long long)(nblocks * 64));
+    return fail(BK_ERR_ARG, msg);
+  }
+  *flops = 2.0 * 64.0 * (double)iters * (double)nblocks * 64.0;
+  BK_HIP(bk::launch_valu_probe(dtype, chains, (uint32_t)nblocks, iters, d_out, S(s)));
+  return BK_OK;
+}
+
 int bk_zero(void *d_dst, size_t bytes, bk_stream s) {
   if (bytes == 0) return BK_OK;
   if (!d_dst) return fail(BK_ERR_ARG, "bk_zero: null pointer");

@@ -129,6 +129,17 @@ int bk_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, bk_stream s);
  * utility with no counterpart in the reference: the benchmark uses it as
  * the achievable-HBM-bandwidth yardstick for the memory-bound sweeps. */
 int bk_stream_copy(void *d_dst, const void *d_src, size_t bytes, bk_stream s);
+/* Register-only multiply-add loop: waves_per_simd one-wave workgroups per
+ * SIMD of the current device, each lane running `iters` iterations of 64
+ * fused multiply-adds (dtype BK_F64 or BK_F32) over `chains` (1 or 8)
+ * independent accumulators with a wave-uniform coefficient, then storing
+ * one value per lane to d_out (capacity out_elems elements: at least
+ * waves_per_simd * CUs * 4 * 64).  *flops receives the launch's flop count
+ * for the caller's own event timing.  A measurement utility with no
+ * counterpart in the reference: the benchmark uses it as the sustained-VALU
+ * yardstick for the compute-bound 125-point cube. */
+int bk_valu_probe(int dtype, int chains, int waves_per_simd, uint32_t iters, void *d_out,
+                  size_t out_elems, double *flops, bk_stream s);
 /* Zero-fill with a kernel (no hipMemset node, so it is safe in graphs). */
 int bk_zero(void *d_dst, size_t bytes, bk_stream s);
 

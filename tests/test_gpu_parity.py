@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from bricklib_amd import BK_F64, stencils as S
+from bricklib_amd import BK_F32, BK_F64, stencils as S
 
 pytestmark = pytest.mark.gpu
 
@@ -280,6 +280,36 @@ def test_stream_copy(gpu):
         _lib.check(lib.bk_stream_copy(C.c_void_p(dd.ptr), C.c_void_p(ds.ptr), nbytes, None))
         np.testing.assert_array_equal(dd.download(np.uint8), src)
     assert lib.bk_stream_copy(C.c_void_p(dd.ptr), C.c_void_p(dd.ptr), 16, None) == 1
+
+
+@pytest.mark.parametrize("dtype,chains", [(BK_F64, 1), (BK_F64, 8), (BK_F32, 1), (BK_F32, 8)])
+def test_valu_probe(gpu, dtype, chains):
+    import ctypes as C
+    from bricklib_amd import _lib
+    lib = gpu.load()
+    n = 256 * 4 * 64  # one wave per SIMD on up to 256 CUs
+    npt = np.float64 if dtype == BK_F64 else np.float32
+    out = gpu.DeviceBuffer(n * np.dtype(npt).itemsize)
+    fl = C.c_double(0)
+    iters = 3
+    _lib.check(lib.bk_valu_probe(dtype, chains, 1, iters, C.c_void_p(out.ptr), n, C.byref(fl),
+                                 None))
+    gpu.device_sync()
+    nblocks = round(fl.value / (2 * 64 * iters * 64))
+    assert fl.value == 2.0 * 64 * iters * nblocks * 64 and nblocks % 4 == 0
+    got = out.download(npt)[:nblocks * 64].astype(np.float64).reshape(nblocks, 64)
+    # every block is the same: lane t sums `chains` accumulators
+    # acc_i = t/1000 + i + iters * (32/chains) * x_i * (0.999 + 1.0001), x_i = (t + i + 1)/2
+    t = np.arange(64, dtype=np.float64)[:, None]
+    i = np.arange(chains, dtype=np.float64)[None, :]
+    want = (t / 1000 + i + iters * (32 / chains) * (t + i + 1) / 2 * (0.999 + 1.0001)).sum(axis=1)
+    np.testing.assert_allclose(got, np.broadcast_to(want, got.shape),
+                               rtol=1e-12 if dtype == BK_F64 else 1e-4)
+    # error paths: too small an output, bad dtype or chain count
+    assert lib.bk_valu_probe(dtype, chains, 1, iters, C.c_void_p(out.ptr), 64, C.byref(fl),
+                             None) == 1
+    assert lib.bk_valu_probe(2, chains, 1, iters, C.c_void_p(out.ptr), n, C.byref(fl), None) == 1
+    assert lib.bk_valu_probe(dtype, 3, 1, iters, C.c_void_p(out.ptr), n, C.byref(fl), None) == 1
 
 
 DEEP = [("7pt", 8), ("13pt", 4), ("25pt", 2), ("125pt", 4), ("cond", 8)]
