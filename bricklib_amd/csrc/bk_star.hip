@@ -59,21 +59,27 @@ __device__ __forceinline__ A tapv(A x) {
 }
 
 // LDS tile of one brick and its R-deep i/j faces: planes k of P x P cells
-// (P = 8 + 2R), element (k, y, x) at at(k, y, x).  For R = 4 (16-wide rows)
-// bit 3 of x is flipped by bit 1 of y xor the plane's parity: the 32 lanes
-// of a read half-wave touch four consecutive rows of 8 consecutive cells,
-// which then fall on distinct banks (rows y and y+2 would otherwise share
-// them), and so do the face stores below -- the 25-point sweep ran 13%
-// (FP64) and 23% (FP32) faster with it.  R = 1, 2 keep P-wide rows: padding
-// them to 16 for the swizzle cost more in occupancy than it saved.
-template <int R>
+// (P = 8 + 2R) of ES bytes, element (k, y, x) at at(k, y, x).  For R = 4
+// (16-wide rows) bit 3 of x is flipped so that the rows one read pass
+// serves land on distinct banks: a pass of 4-byte reads serves 32 lanes =
+// four consecutive rows of 8 cells, and the flip by bit 1 of y (xor the
+// plane's parity) separates rows y and y+2, which a 16-cell stride would put
+// on the same banks; a pass of 8-byte reads serves 16 lanes = two rows, and
+// the flip by bit 0 of y separates them.  The 25-point sweep ran 13% (FP64)
+// and 23% (FP32) faster with the first form, and FP64 another 8% with the
+// second.  R = 1, 2 keep P-wide rows: padding them to 16 for the swizzle
+// cost more in occupancy than it saved.
+template <int R, int ES>
 struct Tile {
   static constexpr int P = 8 + 2 * R;
   static constexpr bool kSwizzle = R == 4;
   static constexpr int RS = kSwizzle ? 16 : P;  // row stride
   static constexpr int PL = P * RS;             // plane stride
+  __device__ static constexpr int flip(int k, int y) {
+    return ES >= 8 ? (y & 1) : (((y >> 1) ^ k) & 1);
+  }
   __device__ static constexpr int at(int k, int y, int x) {
-    return k * PL + y * RS + (kSwizzle ? (x ^ ((((y >> 1) ^ k) & 1) << 3)) : x);
+    return k * PL + y * RS + (kSwizzle ? (x ^ (flip(k, y) << 3)) : x);
   }
 };
 
@@ -104,7 +110,7 @@ struct Faces {
 
   // into planes k0 .. k0+7 of the tile
   __device__ __forceinline__ void store(A *tile, int k0, int lane) const {
-    using TL = Tile<R>;
+    using TL = Tile<R, sizeof(A)>;
     const int c = lane & 7, rr = (lane >> 3) % R, kq = (lane >> 3) / R;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -127,10 +133,14 @@ struct StarBrick {
   static constexpr int P = 8 + 2 * R;
   A v[8], vkm[R], vkp[R];
 
+  // Lane L owns the column (j, i) = (L >> 3, L & 7), element e = L of every
+  // plane.  (The transposed assignment i = L >> 3, j = L & 7 removes the
+  // two-way LDS read conflicts of radius 1 in FP64 but ran the 7-point
+  // sweep 5% slower: lanes then address 64-byte-apart cells.)
   __device__ __forceinline__ void load(const uint32_t *__restrict__ adj, const T *__restrict__ in,
                                        size_t st, uint32_t b, A *tile, int lane) {
-    using TL = Tile<R>;
-    const int j = lane >> 3, i = lane & 7;
+    using TL = Tile<R, sizeof(A)>;
+    const int j = lane >> 3, i = lane & 7, e = lane;
     const uint32_t *ab = adj + (size_t)b * 27;
     const T *pc = in + (size_t)b * st;
     const T *pkm = in + (size_t)uniform(ab[4]) * st;
@@ -140,11 +150,11 @@ struct StarBrick {
     const T *pim = in + (size_t)uniform(ab[12]) * st;
     const T *pip = in + (size_t)uniform(ab[14]) * st;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = widen(pc[k * 64 + lane]);
+    for (int k = 0; k < 8; ++k) v[k] = widen(pc[k * 64 + e]);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      vkm[r] = widen(pkm[(8 - R + r) * 64 + lane]);
-      vkp[r] = widen(pkp[r * 64 + lane]);
+      vkm[r] = widen(pkm[(8 - R + r) * 64 + e]);
+      vkp[r] = widen(pkp[r * 64 + e]);
     }
     Faces<T, R> f;
     f.load(pjm, pjp, pim, pip, lane);
@@ -162,8 +172,8 @@ struct StarBrick {
                                           const uint32_t *__restrict__ imgs = nullptr,
                                           uint32_t nimg = 0) {
     T *__restrict__ po = out + (size_t)b * ost;
-    using TL = Tile<R>;
-    const int j = lane >> 3, i = lane & 7;
+    using TL = Tile<R, sizeof(A)>;
+    const int j = lane >> 3, i = lane & 7, e = lane;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       auto X = [&](int kk) -> A {
@@ -183,9 +193,9 @@ struct StarBrick {
         const bool sel = cond_mode == 0 ? (acc > A(0)) : (v[k] > theta);
         acc = sel ? acc : -acc;
       }
-      store_nt(&po[k * 64 + lane], acc);
+      store_nt(&po[k * 64 + e], acc);
       for (uint32_t q = 0; q < nimg; ++q)
-        store_nt(&out[(size_t)uniform(imgs[q]) * ost + k * 64 + lane], acc);
+        store_nt(&out[(size_t)uniform(imgs[q]) * ost + k * 64 + e], acc);
     }
   }
 };
@@ -193,7 +203,7 @@ struct StarBrick {
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[4][8 * Tile<R>::PL];
+  __shared__ A tile[4][8 * Tile<R, sizeof(A)>::PL];
 
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -231,7 +241,7 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  using TL = Tile<R>;
+  using TL = Tile<R, sizeof(A)>;
   __shared__ A tile[8 * TL::PL];
 
   const int lane = threadIdx.x;
@@ -319,7 +329,7 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_kstack(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  using TL = Tile<R>;
+  using TL = Tile<R, sizeof(A)>;
   constexpr int W = 4;
   __shared__ A tile[W * 8 * TL::PL];
   __shared__ A kb[R][64], ka[R][64];
@@ -440,7 +450,7 @@ hipError_t launch_cols_dt(const SweepLaunch &L, const PlanView &P, hipStream_t s
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_batched(const BatchTable *tab_, uint32_t tick) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[4][8 * Tile<R>::PL];
+  __shared__ A tile[4][8 * Tile<R, sizeof(A)>::PL];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
