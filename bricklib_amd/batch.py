@@ -47,17 +47,42 @@ def _record_dtype():
 REQ_DTYPE = _record_dtype()
 
 
+# star stencils whose batched sweeps march k-columns (the column schedule)
+# instead of one brick per wave.  None by default: the batched column kernel
+# is a persistent grid of one-wave workgroups with a static share of units
+# each, and with 2.5 such waves per SIMD (the radius-4 FP64 tile's LDS
+# limit) the SIMDs holding three finish last -- measured 166 GStencil/s for
+# a 512^3 FP64 25-point grid against 179 one brick per wave (the single-grid
+# column kernel, dispatched unit by unit, reaches 228).
+BATCH_COLUMN_STENCILS: frozenset = frozenset()
+
+
 class Shape:
     """Shared per-shape device tables (adjacency, interior list, ghost
-    pairs, quad plan).  ghost=0: the wrapped-adjacency layout (no ghost
-    bricks, no refresh); ghost=1: the reference's one-brick ghost zone with
-    a periodic refresh before every sweep."""
+    pairs, quad plan, column plan).  ghost=0: the wrapped-adjacency layout
+    (no ghost bricks, no refresh); ghost=1: the reference's one-brick ghost
+    zone with a periodic refresh before every sweep."""
 
     def __init__(self, nb, stream=None, kcol: int = 8, ghost: int = 0):
         self.grid = PeriodicGrid(nb, stream=stream, ghost=ghost)
         self.nb = self.grid.nb
         self.ghost = self.grid.ghost
+        self.kcol = int(kcol)
         self.quad = self.grid.plan("quad", kcol)
+
+    @property
+    def columns(self):
+        """k-marching column plan of the interior bricks (built on first use)."""
+        return self.grid.plan("column", self.kcol)
+
+    def plan_for(self, stencil_kind: int):
+        """The plan a batched request of this stencil sweeps by (None: its
+        bricks one per wave)."""
+        if stencil_kind == BK_CUBE125:
+            return self.quad
+        if stencil_kind in BATCH_COLUMN_STENCILS:
+            return self.columns
+        return None
 
     @property
     def nbricks(self) -> int:
@@ -247,7 +272,8 @@ class Request:
         r.d_adj = g.info.dptr
         r.d_list = g.d_interior.ptr
         r.nlist = g.interior.size
-        r.plan = self.shape.quad.handle if self.stencil.kind == BK_CUBE125 else None
+        pl = self.shape.plan_for(self.stencil.kind)
+        r.plan = pl.handle if pl is not None else None
         r.d_pairs = g.d_pairs.ptr if g.d_pairs is not None else None
         r.npairs = g.npairs
         r.d_f0 = self.f0.ptr

@@ -24,8 +24,15 @@ int plan_fail(int code, const char *msg);
 }
 
 namespace {
-constexpr int kStencils = 5, kDtypes = 3, kGroups = kStencils * kDtypes;
+// sweep groups: (stencil, dtype, schedule), schedule 1 for star requests
+// that come with a column plan (k-marching units instead of single bricks)
+constexpr int kStencils = 5, kDtypes = 3, kGroups = kStencils * kDtypes * 2;
+static_assert(kGroups <= 32, "the non-empty groups form a 32-bit mask");
 constexpr size_t kCoeffSlot = 125;  // doubles per request in the coefficient pool
+
+constexpr int group_of(int stencil, int dtype, bool columns) {
+  return (stencil * kDtypes + dtype) * 2 + (columns ? 1 : 0);
+}
 }  // namespace
 
 struct bk_batch_s {
@@ -72,10 +79,13 @@ hipError_t enqueue_run(bk_batch_s *b, uint32_t nticks, hipStream_t s) {
       }
     for (int g = 0; g < kGroups; ++g)
       if (b->sweep_mask >> g & 1u) {
-        const int stencil = g / kDtypes, dtype = g % kDtypes;
-        hipError_t e = stencil == BK_CUBE125
-                           ? bk::launch_cube_batched(dtype, b->d_sweep[g], t, s)
-                           : bk::launch_star_batched(stencil, dtype, b->d_sweep[g], t, s);
+        const int stencil = g / (2 * kDtypes), dtype = (g / 2) % kDtypes;
+        const bool columns = (g & 1) != 0;
+        hipError_t e =
+            stencil == BK_CUBE125
+                ? bk::launch_cube_batched(dtype, b->d_sweep[g], t, s)
+                : (columns ? bk::launch_star_columns_batched(stencil, dtype, b->d_sweep[g], t, s)
+                           : bk::launch_star_batched(stencil, dtype, b->d_sweep[g], t, s));
         if (e != hipSuccess) return e;
       }
   }
@@ -176,6 +186,11 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
       if (!bk::quad_offsets_fit(bk::plan_view(q.plan).nbricks, q.step))
         return fail(BK_ERR_UNSUPPORTED,
                     "bk_batch_set: the quad schedule addresses at most 2^32 input elements");
+    } else if (q.plan) {
+      int shape = 0;
+      if (bk_plan_query(q.plan, nullptr, nullptr, &shape, nullptr) != BK_OK ||
+          shape != BK_PLAN_COLUMN)
+        return fail(BK_ERR_ARG, "bk_batch_set: a star request's plan must be a BK_PLAN_COLUMN plan");
     } else if (q.nlist && !q.d_list) {
       return fail(BK_ERR_ARG, "bk_batch_set: null brick list");
     }
@@ -188,7 +203,8 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
   for (int d = 0; d < kDtypes; ++d) b->h_ref[d]->prefix[0] = 0;
   for (uint32_t r = 0; r < n; ++r) {
     const bk_batch_req &q = reqs[r];
-    const int g = q.stencil * kDtypes + q.dtype;
+    const bool columns = q.stencil != BK_CUBE125 && q.plan != nullptr;
+    const int g = group_of(q.stencil, q.dtype, columns);
     double *slot = b->h_coeff + (size_t)r * kCoeffSlot;
     const int nc = bk_stencil_ncoeff(q.stencil);
     if (q.dtype == BK_F64) {
@@ -208,7 +224,7 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
     sg.ticks = q.ticks;
     sg.theta = q.theta;
     sg.cond_mode = q.cond_mode;
-    if (q.stencil == BK_CUBE125) {
+    if (q.stencil == BK_CUBE125 || columns) {
       const bk::PlanView P = bk::plan_view(q.plan);
       sg.items = P.anchor;
       sg.len = P.len;

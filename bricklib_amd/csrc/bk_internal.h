@@ -70,9 +70,10 @@ constexpr uint32_t kBatchMaxSegs = 2048;
 
 struct BatchSeg {
   const uint32_t *adj;    // the request's adjacency (its own brick numbering)
-  const uint32_t *items;  // stars: bricks; cube: plan anchors; refresh: pairs
-  const uint32_t *len;    // cube: levels per plan unit
-  const uint32_t *mask;   // cube: store masks [unit][kcol]
+  const uint32_t *items;  // stars: bricks or column-plan anchors; cube: plan anchors;
+                          // refresh: pairs
+  const uint32_t *len;    // plans: bricks (column) or levels (quad) per unit
+  const uint32_t *mask;   // plans: store masks [unit][kcol]
   void *f0;               // the request's two fields
   void *f1;
   const void *coeff;      // coefficients in the accumulate type
@@ -95,6 +96,8 @@ struct BatchTable {
 
 hipError_t launch_star_batched(int stencil, int dtype, const BatchTable *tab, uint32_t tick,
                                hipStream_t s);
+hipError_t launch_star_columns_batched(int stencil, int dtype, const BatchTable *tab,
+                                       uint32_t tick, hipStream_t s);
 hipError_t launch_cube_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
 hipError_t launch_ghost_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
 

@@ -232,47 +232,45 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
 }
 
 
-// Column-marching variant: one wave (one workgroup) per plan unit, a run of
-// up to kcol bricks b, adj[b][22], ... along k.  While brick l is computed,
-// brick l+1 is already being loaded into registers; its plane 0 is the k+1
-// face of brick l, and the top R planes of brick l become the k-1 face of
-// brick l+1, so no k-face is read twice.  i/j faces come through the LDS
-// tile as in the per-brick kernel.  Same per-point arithmetic.
+// Column-marching schedule: one wave per plan unit, a run of up to kcol
+// bricks b, adj[b][22], ... along k.  While brick l is computed, brick l+1
+// is already being loaded into registers; its plane 0 is the k+1 face of
+// brick l, and the top R planes of brick l become the k-1 face of brick
+// l+1, so no k-face is read twice.  i/j faces come through the LDS tile as
+// in the per-brick kernel.  Same per-point arithmetic.  One unit: L bricks
+// from b up the column, store masks mk[0..L); tile is this wave's LDS tile.
+// Shared by the single-grid and the batched kernels.
 template <class T, int R, bool COND>
-__global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, COND> a) {
+__device__ __forceinline__ void column_unit(const uint32_t *__restrict__ adj,
+                                            const T *__restrict__ in, T *__restrict__ out,
+                                            size_t st, size_t ost, uint32_t b, uint32_t L,
+                                            const uint32_t *__restrict__ mk,
+                                            const typename acc_of<T>::type (&c)[6 * R + 1],
+                                            typename acc_of<T>::type theta, int cond_mode,
+                                            typename acc_of<T>::type *tile, int lane) {
   using A = typename acc_of<T>::type;
   using TL = Tile<R, sizeof(A)>;
-  __shared__ A tile[8 * TL::PL];
-
-  const int lane = threadIdx.x;
   const int j = lane >> 3, i = lane & 7;
-  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
-  if (u >= a.nunits) return;
-  const uint32_t L = a.len[u];
-  const uint32_t *mk = a.mask + (size_t)u * a.kcol;
-  const size_t st = a.in_step;
-
-  uint32_t b = a.anchor[u];
   A cur[8], nxt[8], km[R];
   {
-    const T *p = a.in + (size_t)b * st;
+    const T *p = in + (size_t)b * st;
 #pragma unroll
     for (int k = 0; k < 8; ++k) cur[k] = widen(p[k * 64 + lane]);
-    const T *q = a.in + (size_t)a.adj[(size_t)b * 27 + 4] * st;
+    const T *q = in + (size_t)adj[(size_t)b * 27 + 4] * st;
 #pragma unroll
     for (int r = 0; r < R; ++r) km[r] = widen(q[(8 - R + r) * 64 + lane]);
   }
   for (uint32_t l = 0; l < L; ++l) {
-    const uint32_t *ab = a.adj + (size_t)b * 27;
+    const uint32_t *ab = adj + (size_t)b * 27;
     const uint32_t up = ab[22];
     const bool more = l + 1 < L;
     // Face loads first: loads retire in order, so the faces can be waited
     // for while the prefetch of the next brick (issued after) stays in
     // flight through the compute below.
     Faces<T, R> f;
-    f.load(a.in + (size_t)ab[10] * st, a.in + (size_t)ab[16] * st, a.in + (size_t)ab[12] * st,
-           a.in + (size_t)ab[14] * st, lane);
-    const T *pu = a.in + (size_t)up * st;
+    f.load(in + (size_t)ab[10] * st, in + (size_t)ab[16] * st, in + (size_t)ab[12] * st,
+           in + (size_t)ab[14] * st, lane);
+    const T *pu = in + (size_t)up * st;
     if (more) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) nxt[k] = widen(pu[k * 64 + lane]);
@@ -285,24 +283,24 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
     f.store(tile, 0, lane);
     __syncthreads();
     if (mk[l] & 1u) {
-      T *po = a.out + (size_t)b * a.out_step;
+      T *po = out + (size_t)b * ost;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         auto X = [&](int kk) -> A {
           return kk < 0 ? km[kk + R] : (kk >= 8 ? nxt[kk - 8] : cur[kk]);
         };
-        A acc = a.c[0] * tapv<A, COND>(cur[k]);
+        A acc = c[0] * tapv<A, COND>(cur[k]);
 #pragma unroll
         for (int r = 1; r <= R; ++r) {
-          acc = fmadd(a.c[6 * r - 5], tapv<A, COND>(tile[TL::at(k, R + j, R + i + r)]), acc);
-          acc = fmadd(a.c[6 * r - 4], tapv<A, COND>(tile[TL::at(k, R + j, R + i - r)]), acc);
-          acc = fmadd(a.c[6 * r - 3], tapv<A, COND>(tile[TL::at(k, R + j + r, R + i)]), acc);
-          acc = fmadd(a.c[6 * r - 2], tapv<A, COND>(tile[TL::at(k, R + j - r, R + i)]), acc);
-          acc = fmadd(a.c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
-          acc = fmadd(a.c[6 * r], tapv<A, COND>(X(k - r)), acc);
+          acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[TL::at(k, R + j, R + i + r)]), acc);
+          acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[TL::at(k, R + j, R + i - r)]), acc);
+          acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[TL::at(k, R + j + r, R + i)]), acc);
+          acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[TL::at(k, R + j - r, R + i)]), acc);
+          acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
+          acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
         }
         if constexpr (COND) {
-          const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (cur[k] > a.theta);
+          const bool sel = cond_mode == 0 ? (acc > A(0)) : (cur[k] > theta);
           acc = sel ? acc : -acc;
         }
         store_nt(&po[k * 64 + lane], acc);
@@ -316,6 +314,50 @@ __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, CO
     }
     b = up;
     __syncthreads();
+  }
+}
+
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, COND> a) {
+  using A = typename acc_of<T>::type;
+  __shared__ A tile[8 * Tile<R, sizeof(A)>::PL];
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;
+  A c[6 * R + 1];
+#pragma unroll
+  for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
+  column_unit<T, R, COND>(a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], a.len[u],
+                          a.mask + (size_t)u * a.kcol, c, a.theta, a.cond_mode, tile,
+                          threadIdx.x);
+}
+
+// Batched column schedule: one persistent launch serves every request of a
+// group table whose bricks come as a column plan; work item = one plan unit
+// of one request.
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(64) void star_sweep_columns_batched(const BatchTable *tab_,
+                                                                 uint32_t tick) {
+  using A = typename acc_of<T>::type;
+  __shared__ A tile[8 * Tile<R, sizeof(A)>::PL];
+  const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
+  const uint32_t nsegs = tab->nsegs;
+  const uint32_t total = tab->prefix[nsegs];
+  SegCursor cur;
+  for (uint32_t u = xcd_remap(blockIdx.x, gridDim.x); u < total; u += gridDim.x) {
+    cur.seek(tab->prefix, nsegs, u);
+    const auto &sg = tab->seg[cur.s];
+    if (tick >= sg.ticks) continue;
+    const uint32_t ul = u - cur.lo;
+    const bool odd = ((sg.parity + tick) & 1u) != 0;
+    const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
+    T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);
+    const kconst_ptr<A> cp = (kconst_ptr<A>)sg.coeff;
+    A c[6 * R + 1];
+#pragma unroll
+    for (int q = 0; q < 6 * R + 1; ++q) c[q] = cp[q];
+    column_unit<T, R, COND>(sg.adj, in, out, sg.step, sg.step, sg.items[ul], sg.len[ul],
+                            sg.mask + (size_t)ul * sg.kcol, c, (A)sg.theta, sg.cond_mode, tile,
+                            threadIdx.x);
   }
 }
 
@@ -518,6 +560,34 @@ hipError_t launch_batched_dt(int stencil, const BatchTable *tab, uint32_t tick, 
 }
 
 template <class T, int R, bool COND>
+hipError_t launch_cols_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  static int nblk = 0;
+  if (nblk == 0) {
+    int dev = 0, cus = 256, per = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per, star_sweep_columns_batched<T, R, COND>, 64, 0);
+    nblk = cus * (per > 0 ? per : 1);
+  }
+  hipLaunchKernelGGL((star_sweep_columns_batched<T, R, COND>), dim3(nblk), dim3(64), 0, s, tab,
+                     tick);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_cols_batched_dt(int stencil, const BatchTable *tab, uint32_t tick,
+                                  hipStream_t s) {
+  switch (stencil) {
+    case BK_STAR7: return launch_cols_batched_t<T, 1, false>(tab, tick, s);
+    case BK_STAR13: return launch_cols_batched_t<T, 2, false>(tab, tick, s);
+    case BK_STAR25: return launch_cols_batched_t<T, 4, false>(tab, tick, s);
+    case BK_COND7: return launch_cols_batched_t<T, 1, true>(tab, tick, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class T, int R, bool COND>
 hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   using A = typename acc_of<T>::type;
   StarArgs<T, R, COND> a;
@@ -568,6 +638,16 @@ hipError_t launch_star_batched(int stencil, int dtype, const BatchTable *tab, ui
     case BK_F64: return launch_batched_dt<double>(stencil, tab, tick, s);
     case BK_F32: return launch_batched_dt<float>(stencil, tab, tick, s);
     case BK_BF16: return launch_batched_dt<bf16>(stencil, tab, tick, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_star_columns_batched(int stencil, int dtype, const BatchTable *tab,
+                                       uint32_t tick, hipStream_t s) {
+  switch (dtype) {
+    case BK_F64: return launch_cols_batched_dt<double>(stencil, tab, tick, s);
+    case BK_F32: return launch_cols_batched_dt<float>(stencil, tab, tick, s);
+    case BK_BF16: return launch_cols_batched_dt<bf16>(stencil, tab, tick, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -336,8 +336,10 @@ int bk_graph_cache_clear(void);
  * every request with t < ticks refreshes the ghost bricks of its input
  * field (pairs from bk_ghost_pairs, may be none) and sweeps it into the
  * other field: the input is f0 when (parity + t) is even, f1 otherwise.
- * Stars sweep the bricks of d_list; the cube sweeps the bricks of `plan`,
- * which must be a BK_PLAN_QUAD plan.  This is the weak-scaling loop of
+ * Stars sweep the bricks of d_list, or, when `plan` is set (a
+ * BK_PLAN_COLUMN plan), the bricks of that plan by the k-marching column
+ * schedule; the cube sweeps the bricks of `plan`, which must be a
+ * BK_PLAN_QUAD plan.  Either way the results are those of bk_sweep.  This is the weak-scaling loop of
  * weak/main.hip.cpp:243-284 for many subdomains at once, the way the
  * strong-scaling driver's one launch over many subdomains does
  * (strong/main.cu:85-99), with grids of any size, stencil and dtype. */
@@ -345,9 +347,9 @@ typedef struct bk_batch_req {
   int stencil;
   int dtype;
   const uint32_t *d_adj;
-  const uint32_t *d_list; /* stars: bricks to compute */
+  const uint32_t *d_list; /* stars without a plan: bricks to compute */
   uint32_t nlist;
-  bk_plan plan;           /* cube: BK_PLAN_QUAD plan of the bricks */
+  bk_plan plan;           /* cube: BK_PLAN_QUAD plan; stars: BK_PLAN_COLUMN or NULL */
   const uint32_t *d_pairs; /* ghost (dst, src) pairs, or NULL */
   uint32_t npairs;
   void *d_f0;
@@ -361,7 +363,7 @@ typedef struct bk_batch_req {
 } bk_batch_req;
 
 /* A batch serving up to max_reqs requests (<= 2048): one persistent kernel
- * per (stencil, dtype) group and per tick, each over the concatenated work
+ * per (stencil, dtype, schedule) group and per tick, each over the concatenated work
  * of all requests in the group, plus one persistent refresh kernel per
  * dtype.  Allocates its tables (HBM and pinned host memory). */
 int bk_batch_create(uint32_t max_reqs, bk_batch *batch);

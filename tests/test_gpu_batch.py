@@ -106,6 +106,35 @@ def test_poisson_stream_small(gpu):
     assert s["requests"] == 24 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
 
 
+@pytest.mark.parametrize("ghost", [0, 1])
+def test_star_schedules_in_one_batch(gpu, ghost):
+    """Star requests by both batched schedules in one batch -- one brick per
+    wave and k-marching column units (a BK_PLAN_COLUMN plan in the request)
+    -- each give the single-grid result bit for bit, for every star."""
+    shapes = ShapeCache()
+    reqs, q = [], 0
+    for name in ("7pt", "13pt", "25pt", "cond"):
+        for dt in (BK_F64, BK_F32, BK_BF16):
+            for columns in (False, True):
+                nb = ((2, 3, 4), (3, 1, 5), (1, 2, 2), (4, 2, 3))[q % 4]
+                r = Request(rid=q, nb=nb, stencil=Stencil.make(name, seed=200 + q), dtype=dt,
+                            steps=3 + q % 2, seed=300 + q, ghost=ghost)
+                r.allocate(shapes)
+                rq = r.req_struct(0)
+                rq.plan = r.shape.columns.handle if columns else None
+                r._rec = bytes(rq)
+                reqs.append(r)
+                q += 1
+    b = Batch(len(reqs))
+    b.run(reqs, 2)
+    b.run(reqs, 5)
+    for r in reqs:
+        assert r.done == r.steps
+        print(f"[case] rid={r.rid} {r.nb} kind={r.stencil.kind} dtype={r.dtype} "
+              f"steps={r.steps} ghost={ghost}", flush=True)
+        np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
+
+
 def test_malformed_requests_rejected(gpu):
     from bricklib_amd import BrickError
     shapes = ShapeCache()
@@ -122,6 +151,17 @@ def test_malformed_requests_rejected(gpu):
     rq.d_f1 = rq.d_f0  # aliased fields
     arr = (bk_batch_req * 1)(rq)
     assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 1
+    # a star request with a quad plan (stars take column plans only)
+    s = Request(rid=1, nb=(2, 2, 2), stencil=Stencil.make("13pt"), dtype=BK_F64, steps=1,
+                seed=2)
+    s.allocate(shapes)
+    rq = s.req_struct(1)
+    rq.plan = s.shape.quad.handle
+    arr = (bk_batch_req * 1)(rq)
+    assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 1
+    rq.plan = s.shape.columns.handle
+    arr = (bk_batch_req * 1)(rq)
+    assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 0
     with pytest.raises(BrickError):
         Batch(0)
     with pytest.raises(ValueError):
