@@ -231,7 +231,10 @@ def serve_bench(bk, S, args):
             fa = bk.Brick(g.info, g.field_storage(dt), 0)
             fb = bk.Brick(g.info, g.field_storage(dt), 0)
             g.fill_uniform(fa, 3, stream=stream)
-            ms = time_sweeps(bk, st, fa, fb, g, stream, 10, 2)
+            # warmed like the by-stencil cases: a cold GPU would understate
+            # the single-grid rates (the FP64 cube by about a third) and so
+            # overstate the batched efficiency
+            ms = time_sweeps(bk, st, fa, fb, g, stream, 10, 2, warm_s=args.warm_s)
             rates[(S.NAMES[name], dt)] = g.cells / (ms * 1e-3)
             del fa, fb
     del g

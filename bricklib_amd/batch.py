@@ -63,12 +63,16 @@ class Shape:
     (no ghost bricks, no refresh); ghost=1: the reference's one-brick ghost
     zone with a periodic refresh before every sweep."""
 
-    def __init__(self, nb, stream=None, kcol: int = 8, ghost: int = 0):
+    def __init__(self, nb, stream=None, kcol: int = 8, ghost: int = 0, quad_kcol: int = 4):
         self.grid = PeriodicGrid(nb, stream=stream, ghost=ghost)
         self.nb = self.grid.nb
         self.ghost = self.grid.ghost
         self.kcol = int(kcol)
-        self.quad = self.grid.plan("quad", kcol)
+        # 4-level quads: a batch of small grids has few units per persistent
+        # wave, and a tick lasts as long as the waves with one unit more --
+        # 4 levels ran 60 grids of 32..256 cells 20% (FP64) and 11% (FP32)
+        # faster than 8
+        self.quad = self.grid.plan("quad", quad_kcol)
 
     @property
     def columns(self):
