@@ -51,9 +51,9 @@ REQ_DTYPE = _record_dtype()
 # instead of one brick per wave.  None by default: the batched column kernel
 # is a persistent grid of one-wave workgroups with a static share of units
 # each, and with 2.5 such waves per SIMD (the radius-4 FP64 tile's LDS
-# limit) the SIMDs holding three finish last -- measured 166 GStencil/s for
-# a 512^3 FP64 25-point grid against 179 one brick per wave (the single-grid
-# column kernel, dispatched unit by unit, reaches 228).
+# limit) the SIMDs holding three finish last -- on 60 grids of 32..256
+# cells, 25-point FP64 ran 156 GStencil/s by columns against 184 one brick
+# per wave; FP32 362 against 368.
 BATCH_COLUMN_STENCILS: frozenset = frozenset()
 
 
@@ -63,27 +63,31 @@ class Shape:
     (no ghost bricks, no refresh); ghost=1: the reference's one-brick ghost
     zone with a periodic refresh before every sweep."""
 
-    def __init__(self, nb, stream=None, kcol: int = 8, ghost: int = 0, quad_kcol: int = 4):
+    def __init__(self, nb, stream=None, kcol: int = 8, ghost: int = 0):
         self.grid = PeriodicGrid(nb, stream=stream, ghost=ghost)
         self.nb = self.grid.nb
         self.ghost = self.grid.ghost
         self.kcol = int(kcol)
-        # 4-level quads: a batch of small grids has few units per persistent
-        # wave, and a tick lasts as long as the waves with one unit more --
-        # 4 levels ran 60 grids of 32..256 cells 20% (FP64) and 11% (FP32)
-        # faster than 8
-        self.quad = self.grid.plan("quad", quad_kcol)
+        self.quad = self.grid.plan("quad", 4)
 
     @property
     def columns(self):
         """k-marching column plan of the interior bricks (built on first use)."""
         return self.grid.plan("column", self.kcol)
 
-    def plan_for(self, stencil_kind: int):
-        """The plan a batched request of this stencil sweeps by (None: its
-        bricks one per wave)."""
+    def plan_for(self, stencil_kind: int, dtype: int = _lib.BK_F64):
+        """The plan a batched request of this stencil and dtype sweeps by
+        (None: its bricks one per wave).  Cube: quads of 4 levels -- a batch
+        of small grids has few units per persistent wave, and a tick lasts
+        as long as the waves with one unit more: 60 grids of 32..256 cells
+        ran 20% (FP64) and 11% (FP32) faster than with 8 levels, and a
+        lightly loaded stream's latency is lower too (a grid alone in a tick
+        is split into twice as many units).  A device work queue handing
+        units out dynamically was tried: 8% more saturated FP64 throughput
+        with 8-level quads, but a third more latency at light load (its
+        atomics cost every short launch several microseconds)."""
         if stencil_kind == BK_CUBE125:
-            return self.quad
+            return self.grid.plan("quad", 4)
         if stencil_kind in BATCH_COLUMN_STENCILS:
             return self.columns
         return None
@@ -276,7 +280,7 @@ class Request:
         r.d_adj = g.info.dptr
         r.d_list = g.d_interior.ptr
         r.nlist = g.interior.size
-        pl = self.shape.plan_for(self.stencil.kind)
+        pl = self.shape.plan_for(self.stencil.kind, self.dtype)
         r.plan = pl.handle if pl is not None else None
         r.d_pairs = g.d_pairs.ptr if g.d_pairs is not None else None
         r.npairs = g.npairs
