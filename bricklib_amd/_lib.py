@@ -124,6 +124,10 @@ SIGNATURES = {
     "bk_decomp_layout": (C.c_int, [longp, C.c_int, C.c_size_t, C.c_size_t, u32p, u32p,
                                    C.POINTER(C.c_uint32), u32p, C.POINTER(bk_decomp_region),
                                    C.POINTER(C.c_uint32)]),
+    "bk_decomp_layout_skin": (C.c_int, [longp, C.c_int, C.c_size_t, C.c_size_t,
+                                        C.POINTER(C.c_int32), C.c_int, u32p, u32p,
+                                        C.POINTER(C.c_uint32), u32p, C.POINTER(bk_decomp_region),
+                                        C.POINTER(C.c_uint32), u32p]),
     "bk_graph_cache_clear": (C.c_int, []),
     "bk_fill_uniform": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_uint64, C.c_double,
                                   C.c_double, vp]),

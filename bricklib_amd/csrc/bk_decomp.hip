@@ -9,9 +9,11 @@
 //   than a page, else 1), each region's end rounded up to a multiple of
 //   factor:
 //     1. the inner bricks (2g .. nb-1 on every axis, padded coordinates);
-//     2. the skin, region by region in the skin list's order (skin3d_good,
-//        src/brick-mpi.cpp:25-52), a region padded in front when it shares
-//        fewer axes with its predecessor than with its successor;
+//     2. the skin, region by region in the skin list's order (any order of
+//        the 26 regions, as BrickDecomp::initialize takes; bk_decomp_layout
+//        uses skin3d_good, src/brick-mpi.cpp:25-52), a region padded in
+//        front when it shares fewer axes with its predecessor than with its
+//        successor;
 //     3. per neighbour n (26, allneighbors order), the ghost bricks owned by
 //        n, again region by region in skin-list order over the regions that
 //        contain the opposite direction !n -- each run of consecutive such
@@ -55,6 +57,8 @@ const Set kSkinGood[26] = {
     M(0) | M(1),        M(0) | M(1) | P(2), M(1) | P(2),        M(1),
     M(1) | M(2),        P(0) | M(1) | M(2), P(0) | M(1),        P(0) | M(1) | P(2),
     P(0) | P(2),        P(2)};
+
+constexpr int kMaxSkin = 1024;  // bound on skin-list entries (empty ones included)
 
 // the 27 direction sets in allneighbors order: per axis 1..3 (i, j, k),
 // "+", "none", "-", the first axis varying slowest
@@ -111,8 +115,9 @@ struct Layout {
 };
 
 int build(const long nb[3], long g, size_t brick_bytes, size_t page_bytes,
-          std::vector<uint32_t> *grid_out, std::vector<uint32_t> *adj_out, uint32_t *nbricks,
-          uint32_t sep[3], std::vector<bk_decomp_region> *regions) {
+          const std::vector<Set> &skin, std::vector<uint32_t> *grid_out,
+          std::vector<uint32_t> *adj_out, uint32_t *nbricks, uint32_t sep[3],
+          std::vector<bk_decomp_region> *regions, std::vector<uint32_t> *skin_size) {
   Layout L;
   L.g = g;
   long n = 1;
@@ -126,21 +131,23 @@ int build(const long nb[3], long g, size_t brick_bytes, size_t page_bytes,
   L.grid.assign((size_t)n, 0u);
   L.pos = L.factor;
 
-  const int NS = 26;
+  const int NS = (int)skin.size();
   std::vector<bool> pad_first(NS);
   Set last = 0;
   for (int q = 0; q < NS; ++q) {
-    const Set next = q + 1 < NS ? kSkinGood[q + 1] : 0u;
-    pad_first[q] = popc(last & kSkinGood[q]) < popc(kSkinGood[q] & next);
-    last = kSkinGood[q];
+    const Set next = q + 1 < NS ? skin[q + 1] : 0u;
+    pad_first[q] = popc(last & skin[q]) < popc(skin[q] & next);
+    last = skin[q];
   }
   L.populate(0, 0);  // inner bricks
   std::vector<uint32_t> st_pos{L.pos};
   sep[0] = L.pos;
   for (int q = 0; q < NS; ++q) {
-    if (pad_first[q]) L.pos += L.pad(kSkinGood[q]);
-    L.populate(0, kSkinGood[q]);
+    const uint32_t before = L.pos;
+    if (pad_first[q]) L.pos += L.pad(skin[q]);
+    if (skin[q]) L.populate(0, skin[q]);  // an empty entry takes no ids
     st_pos.push_back(L.pos);
+    skin_size->push_back(L.pos - before);  // with the region's padding
   }
   sep[1] = L.pos;
 
@@ -154,25 +161,25 @@ int build(const long nb[3], long g, size_t brick_bytes, size_t page_bytes,
     bool ok = true;
     auto close = [&](int end) {
       r.skin_ed = (uint32_t)end;
-      r.last_pad = pad_first[end - 1] ? 0u : L.pad(kSkinGood[end - 1]);
+      r.last_pad = pad_first[end - 1] ? 0u : L.pad(skin[end - 1]);
       r.len = L.pos - r.ghost_pos;
       // the ghost run and its skin run are laid out alike: equal lengths
       ok = ok && st_pos[end] - r.skin_pos == r.len;
       if (regions) regions->push_back(r);
     };
     for (int q = 0; q < NS; ++q) {
-      if (subset(in, kSkinGood[q])) {
+      if (subset(in, skin[q])) {
         if (run < 0) {
           run = q;
           for (int a = 0; a < 3; ++a)
             r.neighbor[a] = has(nset, a, 1) ? 1 : (has(nset, a, -1) ? -1 : 0);
           r.skin_st = (uint32_t)q;
-          r.first_pad = pad_first[q] ? L.pad(kSkinGood[q]) : 0u;
+          r.first_pad = pad_first[q] ? L.pad(skin[q]) : 0u;
           r.ghost_pos = L.pos;
           r.skin_pos = st_pos[q];
         }
-        if (pad_first[q]) L.pos += L.pad(kSkinGood[q]);
-        L.populate(nset, kSkinGood[q]);
+        if (pad_first[q]) L.pos += L.pad(skin[q]);
+        L.populate(nset, skin[q]);
       } else if (run >= 0) {
         close(q);
         run = -1;
@@ -211,10 +218,12 @@ int build(const long nb[3], long g, size_t brick_bytes, size_t page_bytes,
 
 extern "C" {
 
-int bk_decomp_layout(const long nb[3], int ghost_bricks, size_t brick_bytes, size_t page_bytes,
-                     uint32_t *h_grid, uint32_t *h_adj, uint32_t *nbricks, uint32_t sep_pos[3],
-                     bk_decomp_region *h_regions, uint32_t *nregions) {
-  if (!nb || !nbricks || !nregions || !sep_pos)
+int bk_decomp_layout_skin(const long nb[3], int ghost_bricks, size_t brick_bytes,
+                          size_t page_bytes, const int32_t *h_skin, int nskin, uint32_t *h_grid,
+                          uint32_t *h_adj, uint32_t *nbricks, uint32_t sep_pos[3],
+                          bk_decomp_region *h_regions, uint32_t *nregions,
+                          uint32_t *h_skin_size) {
+  if (!nb || !nbricks || !nregions || !sep_pos || !h_skin)
     return bk::plan_fail(BK_ERR_ARG, "bk_decomp_layout: null pointer");
   if (ghost_bricks < 1) return bk::plan_fail(BK_ERR_ARG, "bk_decomp_layout: ghost_bricks >= 1");
   for (int a = 0; a < 3; ++a)
@@ -228,19 +237,57 @@ int bk_decomp_layout(const long nb[3], int ghost_bricks, size_t brick_bytes, siz
   const double cells = (double)(nb[0] + 2 * ghost_bricks) * (nb[1] + 2 * ghost_bricks) *
                        (nb[2] + 2 * ghost_bricks);
   if (cells > 1.0e9) return bk::plan_fail(BK_ERR_ARG, "bk_decomp_layout: grid too large");
-  std::vector<uint32_t> grid, adj;
+  if (nskin < 26 || nskin > kMaxSkin)
+    return bk::plan_fail(BK_ERR_ARG, "bk_decomp_layout: the skin list needs 26 to 1024 entries");
+  // the surface order: the 26 non-empty direction sets once each, empty
+  // entries anywhere (they take no ids but break runs and move padding)
+  std::vector<Set> skin(nskin);
+  uint64_t seen = 0;  // bit r: direction set r already listed
+  int nonempty = 0;
+  for (int q = 0; q < nskin; ++q) {
+    Set r = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int32_t v = h_skin[3 * q + a];
+      if (v == 1) r |= P(a);
+      else if (v == -1) r |= M(a);
+      else if (v != 0)
+        return bk::plan_fail(BK_ERR_ARG, "bk_decomp_layout: skin components must be -1, 0 or 1");
+    }
+    if (r != 0) {
+      if (seen >> r & 1u)
+        return bk::plan_fail(BK_ERR_ARG, "bk_decomp_layout: a skin region is listed twice");
+      seen |= uint64_t(1) << r;
+      ++nonempty;
+    }
+    skin[q] = r;
+  }
+  if (nonempty != 26)
+    return bk::plan_fail(BK_ERR_ARG, "bk_decomp_layout: the skin list must name all 26 regions");
+  std::vector<uint32_t> grid, adj, sizes;
   std::vector<bk_decomp_region> regions;
   uint32_t nbr = 0, sep[3];
-  int rc = build(nb, ghost_bricks, brick_bytes, page_bytes, h_grid ? &grid : nullptr,
-                 h_adj ? &adj : nullptr, &nbr, sep, &regions);
+  int rc = build(nb, ghost_bricks, brick_bytes, page_bytes, skin, h_grid ? &grid : nullptr,
+                 h_adj ? &adj : nullptr, &nbr, sep, &regions, &sizes);
   if (rc != BK_OK) return rc;
   if (h_grid) std::memcpy(h_grid, grid.data(), grid.size() * sizeof(uint32_t));
   if (h_adj) std::memcpy(h_adj, adj.data(), adj.size() * sizeof(uint32_t));
   if (h_regions) std::memcpy(h_regions, regions.data(), regions.size() * sizeof(regions[0]));
+  if (h_skin_size) std::memcpy(h_skin_size, sizes.data(), sizes.size() * sizeof(uint32_t));
   *nbricks = nbr;
   *nregions = (uint32_t)regions.size();
   for (int q = 0; q < 3; ++q) sep_pos[q] = sep[q];
   return BK_OK;
+}
+
+int bk_decomp_layout(const long nb[3], int ghost_bricks, size_t brick_bytes, size_t page_bytes,
+                     uint32_t *h_grid, uint32_t *h_adj, uint32_t *nbricks, uint32_t sep_pos[3],
+                     bk_decomp_region *h_regions, uint32_t *nregions) {
+  int32_t skin[26 * 3];
+  for (int q = 0; q < 26; ++q)
+    for (int a = 0; a < 3; ++a)
+      skin[3 * q + a] = has(kSkinGood[q], a, 1) ? 1 : (has(kSkinGood[q], a, -1) ? -1 : 0);
+  return bk_decomp_layout_skin(nb, ghost_bricks, brick_bytes, page_bytes, skin, 26, h_grid,
+                               h_adj, nbricks, sep_pos, h_regions, nregions, nullptr);
 }
 
 }  // extern "C"

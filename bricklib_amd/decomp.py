@@ -1,10 +1,11 @@
 """The BrickDecomp layout (bk_decomp_layout) and a periodic grid in it.
 
 BrickDecomp (include/brick-mpi.h:178-713) numbers a subdomain's bricks as
-inner bricks, then the skin in the skin3d_good order, then the ghost bricks
-of each of the 26 neighbours, so that every exchange moves contiguous runs
-of bricks.  :class:`DecompLayout` holds that numbering; :class:`DecompGrid`
-is a one-subdomain periodic grid in it, usable wherever a PeriodicGrid is
+inner bricks, then the skin in a surface order (skin3d_good by default),
+then the ghost bricks of each of the 26 neighbours, so that every exchange
+moves contiguous runs of bricks.  :class:`DecompLayout` holds that
+numbering; :class:`DecompGrid` is a one-subdomain periodic grid in it,
+usable wherever a PeriodicGrid is
 (sweeps, plans, the fused and deep-ghost forms): its ghost refresh copies
 each neighbour's ghost run from the matching skin run.
 """
@@ -19,24 +20,72 @@ from ._lib import bk_decomp_region, check, load, longs
 from .bricks import (BrickInfo, DeviceBuffer, PeriodicGrid, movBrickInfo)
 
 
-class DecompLayout:
-    """bk_decomp_layout for nb bricks per axis (i, j, k), ghost bricks deep."""
+def _lexicographic():
+    """The 27 regions in (k, j, i) lexicographic order, -1 first, i fastest,
+    as signed axis numbers (1 = i), the empty centre included."""
+    return [tuple(v for v in (i, 2 * j, 3 * k) if v)
+            for k in (-1, 0, 1) for j in (-1, 0, 1) for i in (-1, 0, 1)]
 
-    def __init__(self, nb, ghost: int = 1, brick_bytes: int = 4096, page_bytes: int = 4096):
+
+# The surface orders of src/brick-mpi.cpp:25-136 in the reference's BitSet
+# notation: an entry lists signed axis numbers, (1, -3) = +i and -k.
+SKIN3D_GOOD = [(1,), (1, -3), (1, 2, -3), (1, 2), (1, 2, 3), (2, 3), (2,), (2, -3),
+               (-1, 2, -3), (-1, 2), (-1, 2, 3), (-1, 3), (-1,), (-3,), (-1, -3),
+               (-1, -2, -3), (-1, -2), (-1, -2, 3), (-2, 3), (-2,), (-2, -3), (1, -2, -3),
+               (1, -2), (1, -2, 3), (1, 3), (3,)]
+SKIN3D_NORMAL = _lexicographic()
+SKIN3D_BAD = [e for r in _lexicographic() if r for e in ((), r)][1:]
+
+
+def encode_skin(skin) -> np.ndarray:
+    """Skin entries in BitSet notation -> int32 (i, j, k) triples of -1/0/+1
+    (the encoding of bk_decomp_layout_skin)."""
+    out = np.zeros((len(skin), 3), np.int32)
+    for q, entry in enumerate(skin):
+        for v in entry:
+            a = abs(int(v)) - 1
+            if not 0 <= a < 3 or out[q, a] != 0:
+                raise ValueError(f"bad skin entry {entry!r}")
+            out[q, a] = 1 if v > 0 else -1
+    return out
+
+
+class DecompLayout:
+    """bk_decomp_layout for nb bricks per axis (i, j, k), ghost bricks deep,
+    in the surface order `skin` (None: skin3d_good through
+    bk_decomp_layout; else a list in BitSet notation, see SKIN3D_GOOD,
+    through bk_decomp_layout_skin)."""
+
+    def __init__(self, nb, ghost: int = 1, brick_bytes: int = 4096, page_bytes: int = 4096,
+                 skin=None):
         self.nb = tuple(int(x) for x in nb)
         self.g = int(ghost)
         self.padded = tuple(x + 2 * self.g for x in self.nb)
         nbr, nreg = C.c_uint32(), C.c_uint32()
         sep = (C.c_uint32 * 3)()
         args = (longs(self.nb), self.g, C.c_size_t(brick_bytes), C.c_size_t(page_bytes))
-        check(load().bk_decomp_layout(*args, None, None, C.byref(nbr), sep, None, C.byref(nreg)))
+        lib = load()
+        if skin is None:
+            def call(grid, adj, regs, sizes):
+                return lib.bk_decomp_layout(*args, grid, adj, C.byref(nbr), sep, regs,
+                                            C.byref(nreg))
+        else:
+            enc = encode_skin(skin)
+
+            def call(grid, adj, regs, sizes):
+                return lib.bk_decomp_layout_skin(*args, enc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                 len(enc), grid, adj, C.byref(nbr), sep, regs,
+                                                 C.byref(nreg), sizes)
+        check(call(None, None, None, None))
         cells = int(np.prod(self.padded))
         grid = np.zeros(cells, np.uint32)
         adj = np.zeros(nbr.value * 27, np.uint32)
         regs = (bk_decomp_region * max(1, nreg.value))()
-        check(load().bk_decomp_layout(*args, grid.ctypes.data_as(_lib.u32p),
-                                      adj.ctypes.data_as(_lib.u32p), C.byref(nbr), sep, regs,
-                                      C.byref(nreg)))
+        sizes = np.zeros(max(1, len(skin or ())), np.uint32)
+        check(call(grid.ctypes.data_as(_lib.u32p), adj.ctypes.data_as(_lib.u32p), regs,
+                   sizes.ctypes.data_as(_lib.u32p)))
+        #: ids per skin-list entry, padding included (given a skin list)
+        self.skin_size = sizes[:len(skin)].tolist() if skin is not None else None
         self.nbricks = nbr.value
         self.sep = tuple(int(x) for x in sep)
         self.grid = grid.reshape(self.padded[2], self.padded[1], self.padded[0])
@@ -65,10 +114,11 @@ class DecompGrid(PeriodicGrid):
     """A periodic grid of nb bricks per axis with a one-brick ghost zone in
     the BrickDecomp numbering.  Fields have nbricks (the layout's id range)
     chunks; the refresh copies the 26 neighbours' ghost runs from their
-    skin runs.  Sweep order: grid order over the interior."""
+    skin runs.  Sweep order: grid order over the interior.  `skin`: the
+    surface order (BitSet notation, None = skin3d_good)."""
 
-    def __init__(self, nb, stream=None, brick_bytes: int = 4096):
-        self.layout = L = DecompLayout(nb, 1, brick_bytes)
+    def __init__(self, nb, stream=None, brick_bytes: int = 4096, skin=None):
+        self.layout = L = DecompLayout(nb, 1, brick_bytes, skin=skin)
         self.nb = L.nb
         self.ghost = 1
         self.padded = L.padded

@@ -208,6 +208,21 @@ int bk_decomp_layout(const long nb[3], int ghost_bricks, size_t brick_bytes,
                      size_t page_bytes, uint32_t *h_grid, uint32_t *h_adj,
                      uint32_t *nbricks, uint32_t sep_pos[3],
                      bk_decomp_region *h_regions, uint32_t *nregions);
+/* The same layout for any surface order, as BrickDecomp::initialize takes
+ * a skin list (brick-mpi.h:322; the reference ships skin3d_good,
+ * skin3d_normal and skin3d_bad, src/brick-mpi.cpp:25-110).  h_skin holds
+ * nskin entries in order, three int32 per entry, one per axis (i, j, k):
+ * +1, -1 or 0 (the reference's BitSet {1, -3} is (+1, 0, -1)).  Each of
+ * the 26 non-empty regions must appear once; empty entries (0, 0, 0) may
+ * appear anywhere, as in skin3d_normal and skin3d_bad: they take no ids but
+ * end runs and move padding, as in the reference.  26 <= nskin <= 1024,
+ * BK_ERR_ARG otherwise.  h_skin_size (nskin entries, may be NULL) receives
+ * the reference's skin_size: the ids each entry takes, padding included. */
+int bk_decomp_layout_skin(const long nb[3], int ghost_bricks, size_t brick_bytes,
+                          size_t page_bytes, const int32_t *h_skin, int nskin, uint32_t *h_grid,
+                          uint32_t *h_adj, uint32_t *nbricks, uint32_t sep_pos[3],
+                          bk_decomp_region *h_regions, uint32_t *nregions,
+                          uint32_t *h_skin_size);
 /* Interior brick list in grid order (k slowest, i fastest) for the same
  * padded grid: the bricks the sweep loops of stencils/3axis.cpp:51-66 visit.
  * h_list may be NULL to query *n. */

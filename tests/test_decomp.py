@@ -6,7 +6,7 @@ is the periodic image of its skin run, brick by brick."""
 import numpy as np
 import pytest
 
-from bricklib_amd.decomp import DecompLayout
+from bricklib_amd.decomp import SKIN3D_BAD, SKIN3D_GOOD, SKIN3D_NORMAL, DecompLayout
 
 
 def coords(L):
@@ -31,10 +31,64 @@ def test_two_cubed_follows_skin_order():
     assert L.sep == (1, 9, 65) and L.nbricks == 65
 
 
+def test_skin3d_good_list_equals_default():
+    a = DecompLayout((4, 3, 5), brick_bytes=2048)
+    b = DecompLayout((4, 3, 5), brick_bytes=2048, skin=SKIN3D_GOOD)
+    np.testing.assert_array_equal(a.grid, b.grid)
+    np.testing.assert_array_equal(a.adj, b.adj)
+    assert a.regions == b.regions and a.sep == b.sep
+
+
+def test_skin3d_normal_three_cubed():
+    # nb = 3, one brick per region: the inner brick first (id 1), then the
+    # 26 regions in lexicographic order (k slowest, -1 first), the empty
+    # centre entry taking no id
+    L = DecompLayout((3, 3, 3), skin=SKIN3D_NORMAL)
+    for k in range(1, 4):
+        for j in range(1, 4):
+            for i in range(1, 4):
+                r = 9 * (k - 1) + 3 * (j - 1) + (i - 1)
+                want = 1 if r == 13 else (2 + r if r < 13 else 1 + r)
+                assert L.grid[k, j, i] == want, (i, j, k)
+    assert L.sep == (2, 28, 126)
+    assert L.skin_size == [1] * 13 + [0] + [1] * 13
+
+
+def test_skin3d_bad_groups_pages_per_region():
+    # nb = 2, 2048-byte bricks (two to a 4096-byte page): only corners have
+    # bricks; each is alone between empty entries, so each takes a page:
+    # interior ids 2, 4, .., 16 in lexicographic order, and each of the
+    # 8 x 7 ghost bricks (a corner seen from each neighbour that sees it)
+    # its own page: 18 + 2 * 56 = 130
+    L = DecompLayout((2, 2, 2), brick_bytes=2048, skin=SKIN3D_BAD)
+    ids = L.grid[1:3, 1:3, 1:3].reshape(-1)
+    np.testing.assert_array_equal(ids, np.arange(2, 18, 2))
+    assert L.sep == (2, 18, 130) and L.nbricks == 130
+    assert sum(L.skin_size) == 16 and L.skin_size[1::2] == [0] * 25
+    # every region is its own run: 26 neighbours x the regions containing
+    # the opposite direction (9 per face, 3 per edge, 1 per corner)
+    assert len(L.regions) == 6 * 9 + 12 * 3 + 8 * 1
+
+
+def test_rejects_bad_skin_lists():
+    from bricklib_amd import BrickError
+    with pytest.raises(BrickError):
+        DecompLayout((4, 4, 4), skin=SKIN3D_GOOD[:-1])  # a region missing
+    with pytest.raises(BrickError):
+        DecompLayout((4, 4, 4), skin=SKIN3D_GOOD[:-1] + [(1,)])  # a region twice
+    with pytest.raises(ValueError):
+        DecompLayout((4, 4, 4), skin=SKIN3D_GOOD[:-1] + [(4,)])  # no axis 4
+
+
+SKINS = {"good": None, "normal": SKIN3D_NORMAL, "bad": SKIN3D_BAD,
+         "shuffled": [SKIN3D_GOOD[q] for q in np.random.default_rng(5).permutation(26)]}
+
+
+@pytest.mark.parametrize("skin", sorted(SKINS))
 @pytest.mark.parametrize("nb,bb", [((2, 2, 2), 4096), ((4, 3, 5), 4096), ((6, 6, 6), 4096),
                                    ((4, 3, 5), 2048), ((5, 4, 2), 1024)])
-def test_sections_adjacency_and_exchange(nb, bb):
-    L = DecompLayout(nb, 1, brick_bytes=bb)
+def test_sections_adjacency_and_exchange(nb, bb, skin):
+    L = DecompLayout(nb, 1, brick_bytes=bb, skin=SKINS[skin])
     factor = max(1, 4096 // bb)
     C = coords(L)
     ids = np.array(sorted(C))

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from bricklib_amd import BK_F32, BK_F64, stencils as S
-from bricklib_amd.decomp import DecompGrid
+from bricklib_amd.decomp import SKIN3D_BAD, SKIN3D_NORMAL, DecompGrid
 
 pytestmark = pytest.mark.gpu
 
@@ -32,3 +32,15 @@ def test_decomp_matches_init_grid_layout(gpu, name):
             np.testing.assert_array_equal(run(gpu, dg, st, dt, 3, plan=None, fused=True), want)
         gi = 8 // S.RADIUS[st.kind]
         np.testing.assert_array_equal(run(gpu, dg, st, dt, 3, ghost_iters=min(gi, 3)), want)
+
+
+@pytest.mark.parametrize("skin", ["normal", "bad"])
+def test_other_skin_orders_match(gpu, skin):
+    """The reference's other surface orders number the bricks differently;
+    the sweeps over them give the same values bit for bit."""
+    nb = (3, 4, 5)
+    st = S.Stencil.make("13pt", seed=7)
+    dg = DecompGrid(nb, skin={"normal": SKIN3D_NORMAL, "bad": SKIN3D_BAD}[skin])
+    pg = gpu.PeriodicGrid(nb, tile=0)
+    print(f"[case] decomp skin={skin}", flush=True)
+    np.testing.assert_array_equal(run(gpu, dg, st, BK_F64, 4), run(gpu, pg, st, BK_F64, 4))

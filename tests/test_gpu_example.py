@@ -20,3 +20,20 @@ def test_drop_in_example(gpu, n, sweeps):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "result match" in r.stdout
+
+
+DECOMP_EXE = os.path.join(ROOT, "examples", "bin", "decomp_7pt")
+
+
+@pytest.mark.parametrize("n,sweeps,skin", [(32, 4, "good"), (24, 3, "normal"), (16, 5, "bad")])
+def test_decomp_example(gpu, n, sweeps, skin):
+    """BrickDecomp (include/brick-decomp.h) on device storage: exchange() +
+    brick_sweep per step, and brick_sweeps over device_pairs(), against the
+    host array loop; the two device paths bit-identical."""
+    assert os.path.exists(DECOMP_EXE), "examples/bin/decomp_7pt not built (run build())"
+    print(f"[case] decomp_7pt n={n} sweeps={sweeps} skin={skin}", flush=True)
+    r = subprocess.run([DECOMP_EXE, str(n), str(sweeps), skin], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "bit-identical" in r.stdout and "result match" in r.stdout
