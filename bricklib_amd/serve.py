@@ -9,6 +9,14 @@ between chunks -- requests that arrive while others are on the GPU join the
 next chunk, so their work overlaps on the device.  Latency of a request =
 completion time - arrival time; throughput = interior cells x sweeps of all
 completed requests / elapsed time.
+
+The admitted requests are split over `lanes` batches, each on its own
+stream, joined once per chunk: lane 0 takes the FP64 cubes and the other
+dtypes' stars, lane 1 the other cubes and the FP64 stars, so a lane's
+compute-bound cube ticks run beside the other lane's memory-bound star
+ticks and fill the tails of each other's persistent kernels (the
+saturating mix of the benchmark: 385 -> 357 ms on one MI355X; both cubes
+on one lane and the stars on the other gained nothing).
 """
 from __future__ import annotations
 
@@ -17,9 +25,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import BK_F32, BK_F64
+from ._lib import BK_CUBE125, BK_F32, BK_F64
 from .batch import Arena, Batch, Request, ShapeCache
-from .bricks import Event, Stream
+from .bricks import Stream
 from .stencils import NAMES, Stencil, u01
 
 
@@ -71,16 +79,31 @@ class ServeStats:
                 "chunks": self.chunks}
 
 
+def lane_of(kind: int, dtype: int, lanes: int) -> int:
+    """The lane of a request with `lanes` lanes: 0 for FP64 cubes and the
+    stars (and cond) of the other dtypes, 1 for the other cubes and the
+    FP64 stars; 0 with one lane."""
+    if lanes == 1:
+        return 0
+    return 0 if (kind == BK_CUBE125) == (dtype == BK_F64) else 1
+
+
 class Server:
     """Continuous-batching loop over one Batch."""
 
     def __init__(self, hbm_budget: float = 200e9, chunk: int = 8, max_active: int = 2048,
-                 use_graph: bool = True, ghost: int = 0, slab_bytes: int = 1 << 30):
-        self.stream = Stream()
-        # request fields come from an arena used only on self.stream: a
-        # retired request's memory is reused at once, without a driver call
+                 use_graph: bool = True, ghost: int = 0, slab_bytes: int = 1 << 30,
+                 lanes: int = 2):
+        if lanes not in (1, 2):
+            raise ValueError("lanes must be 1 or 2")
+        self.streams = [Stream() for _ in range(lanes)]
+        self.stream = self.streams[0]
+        # request fields come from an arena; a retired request's memory is
+        # reused at once, without a driver call: blocks are released only
+        # after every lane's stream has finished the chunk
         self.arena = Arena(slab_bytes)
-        self.batch = Batch(max_active, stream=self.stream)
+        self.batches = [Batch(max_active, stream=s) for s in self.streams]
+        self.batch = self.batches[0]
         self.shapes = ShapeCache()
         self.budget = float(hbm_budget)
         self.chunk = int(chunk)
@@ -89,6 +112,9 @@ class Server:
         # field layout of admitted requests: 0 = wrapped adjacency (no ghost
         # bricks, no refresh), 1 = one-brick ghost zone refreshed every tick
         self.ghost = int(ghost)
+
+    def lane(self, r: Request) -> int:
+        return lane_of(r.stencil.kind, r.dtype, len(self.batches))
 
     def admissible(self, r: Request, used: int, nactive: int) -> bool:
         return nactive < self.max_active and used + r.hbm_bytes <= self.budget
@@ -119,7 +145,6 @@ class Server:
         st = ServeStats()
         t0 = time.perf_counter()
         qi = 0
-        ev = Event()
         while qi < len(pending) or active:
             now = time.perf_counter() - t0
             # admit arrived requests that fit
@@ -127,7 +152,8 @@ class Server:
                 r = pending[qi]
                 if not self.admissible(r, used, len(active)):
                     break
-                r.allocate(self.shapes, self.stream, self.arena)
+                # filled on its lane's stream, which runs all its ticks
+                r.allocate(self.shapes, self.streams[self.lane(r)], self.arena)
                 r.admitted = now
                 used += r.hbm_bytes
                 active.append(r)
@@ -138,9 +164,14 @@ class Server:
                     time.sleep(max(0.0, pending[qi].arrival - (time.perf_counter() - t0)))
                 continue
             ticks = min(self.chunk, max(r.remaining for r in active))
-            self.batch.run(active, ticks, self.use_graph)
-            ev.record(self.stream)
-            ev.sync()
+            lanes = [[] for _ in self.batches]
+            for r in active:
+                lanes[self.lane(r)].append(r)
+            for b, reqs in zip(self.batches, lanes):
+                if reqs:
+                    b.run(reqs, ticks, self.use_graph)
+            for s in self.streams:
+                s.sync()
             st.chunks += 1
             now = time.perf_counter() - t0
             still = []

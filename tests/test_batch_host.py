@@ -35,3 +35,14 @@ def test_packed_records_round_trip():
             0x1000 + q, 0x2000 + q, 0x3000 + q, 0x4000 + q)
         assert p[q].h_coeff[6] == 6 + q and p[q].theta == 0.25 * q
         assert (p[q].ticks, p[q].parity) == (5 + q, (q + 1) % 2)
+
+
+def test_serving_lanes_pair_cubes_with_the_other_dtypes_stars():
+    from bricklib_amd._lib import (BK_BF16, BK_COND7, BK_CUBE125, BK_F32, BK_F64, BK_STAR7,
+                                   BK_STAR25)
+    from bricklib_amd.serve import lane_of
+    assert lane_of(BK_CUBE125, BK_F64, 2) == 0
+    assert lane_of(BK_STAR7, BK_F32, 2) == 0 and lane_of(BK_STAR25, BK_BF16, 2) == 0
+    assert lane_of(BK_CUBE125, BK_F32, 2) == 1
+    assert lane_of(BK_STAR7, BK_F64, 2) == 1 and lane_of(BK_COND7, BK_F64, 2) == 1
+    assert all(lane_of(k, d, 1) == 0 for k in range(5) for d in (BK_F64, BK_F32, BK_BF16))

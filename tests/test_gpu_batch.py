@@ -41,15 +41,20 @@ def alone(gpu, r: Request, ghost: int = 1) -> np.ndarray:
     return res.storage.to_host()[g.interior]
 
 
-@pytest.mark.parametrize("chunk,ghost", [(1, 0), (3, 0), (8, 0), (3, 1), (8, 1)])
-def test_mix_matches_alone(gpu, chunk, ghost):
-    srv = serve.Server(chunk=chunk, ghost=ghost)
+@pytest.mark.parametrize("chunk,ghost,lanes", [(1, 0, 2), (3, 0, 2), (8, 0, 2), (3, 1, 2),
+                                               (8, 1, 2), (3, 0, 1), (8, 1, 1)])
+def test_mix_matches_alone(gpu, chunk, ghost, lanes):
+    """Every request of a mixed batch -- on one stream, or split over two
+    lanes that run concurrently -- gives its single-grid result bit for bit."""
+    srv = serve.Server(chunk=chunk, ghost=ghost, lanes=lanes)
     reqs = make_mix()
+    if lanes == 2:
+        assert {srv.lane(r) for r in reqs} == {0, 1}
     stats = srv.serve(reqs, realtime=False, keep_results=True)
     assert len(stats.completed) == len(reqs)
     for r in reqs:
         print(f"[case] rid={r.rid} {r.nb} kind={r.stencil.kind} dtype={r.dtype} "
-              f"steps={r.steps} ghost={ghost}", flush=True)
+              f"steps={r.steps} ghost={ghost} lanes={lanes}", flush=True)
         np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
 
 
