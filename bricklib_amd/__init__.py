@@ -6,6 +6,17 @@ Python host mirror (ctypes), used by the tests, the benchmark and the
 batched/serving runtimes.  Importing it does not load the library; the
 first call that needs it does, and fails loudly when it is not built.
 """
+import os as _os
+
+# HIP gives a process GPU_MAX_HW_QUEUES hardware queues (4 unless set) and
+# shares them among its streams beyond that count.  The serving loop runs
+# two streams concurrently; with four queues, whether they land on separate
+# queues depends on how many streams the process already holds (measured:
+# one stream more, and the two lanes ran one after the other, p50 latency
+# 10 -> 13.5 ms).  Eight queues keep them apart.  Takes effect only when
+# HIP is initialised after this import; a value set by the user is kept.
+_os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 from ._lib import (BK_BF16, BK_COND7, BK_COND_REFERENCE, BK_COND_THRESHOLD, BK_CUBE125,
                    BK_F32, BK_F64, BK_STAR7, BK_STAR13, BK_STAR25, BrickError, load)
 from .bricks import (Brick, BrickInfo, BrickStorage, DeviceBuffer, Event, PeriodicGrid,
