@@ -22,7 +22,10 @@ The same JSON line also carries:
                   plus the FP32 / BF16 forms of the 7-point star;
   precision    -- FP32 and BF16-storage forms against the FP64 sweep after
                   1, 10 and 100 sweeps at 512^3 and 1024^3 (errors computed
-                  in FP64 on the GPU) and their speed-up over FP64;
+                  in FP64 on the GPU) and their speed-up over FP64 per step
+                  (separate refresh kernel each sweep; for the 7-point star
+                  also with the refresh fused into the sweep, the headline's
+                  form);
   serving      -- continuous batching of independent solves (edges 32..256,
                   7/13/25/125-point, FP64/FP32, 1..200 sweeps): Poisson
                   arrivals at 200 req/s (p50/p99 latency), a saturating run
@@ -430,17 +433,27 @@ def main():
                 prec[key] = PR.study(pn, sname, dtypes=dts, sweeps=(1, 10, 100), stream=stream)
                 gp = bk.PeriodicGrid((pn // 8,) * 3, stream=stream)
                 stp = S.Stencil.make(sname, seed=7)
-                times = {}
+                times, fused = {}, {}
                 for dt in (bk.BK_F64, bk.BK_F32) + ((bk.BK_BF16,) if sname == "7pt" else ()):
                     fa = bk.Brick(gp.info, gp.field_storage(dt), 0)
                     fb = bk.Brick(gp.info, gp.field_storage(dt), 0)
                     gp.fill_uniform(fa, 7, stream=stream)
                     times[dt] = time_sweeps(bk, stp, fa, fb, gp, stream, 20, 2,
                                             warm_s=args.warm_s)
+                    if sname == "7pt":
+                        # the headline's form: the refresh folded into the
+                        # sweep's stores (stars only)
+                        fused[dt] = time_sweeps(bk, stp, fa, fb, gp, stream, 20, 2,
+                                                warm_s=args.warm_s / 2, fused=True)
                     del fa, fb
                 prec[key]["ms_per_step"] = {PR.DTYPE_NAMES[d]: round(t, 4) for d, t in times.items()}
                 prec[key]["speedup_vs_f64"] = {PR.DTYPE_NAMES[d]: round(times[bk.BK_F64] / t, 3)
                                                for d, t in times.items()}
+                if fused:
+                    prec[key]["ms_per_step_fused_refresh"] = {
+                        PR.DTYPE_NAMES[d]: round(t, 4) for d, t in fused.items()}
+                    prec[key]["speedup_vs_f64_fused_refresh"] = {
+                        PR.DTYPE_NAMES[d]: round(fused[bk.BK_F64] / t, 3) for d, t in fused.items()}
                 del gp
                 print(f"[bench] precision {key}: {prec[key]}", file=sys.stderr, flush=True)
 

@@ -46,3 +46,22 @@ def test_serving_lanes_pair_cubes_with_the_other_dtypes_stars():
     assert lane_of(BK_CUBE125, BK_F32, 2) == 1
     assert lane_of(BK_STAR7, BK_F64, 2) == 1 and lane_of(BK_COND7, BK_F64, 2) == 1
     assert all(lane_of(k, d, 1) == 0 for k in range(5) for d in (BK_F64, BK_F32, BK_BF16))
+
+
+def test_import_raises_the_hardware_queue_default_but_keeps_a_users_value():
+    """The serving lanes need their streams on separate hardware queues;
+    importing the package sets GPU_MAX_HW_QUEUES=8 unless it is set."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = "import os, bricklib_amd; print(os.environ['GPU_MAX_HW_QUEUES'])"
+    env = {k: v for k, v in os.environ.items() if k != "GPU_MAX_HW_QUEUES"}
+    env["PYTHONPATH"] = root
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         check=True).stdout.strip()
+    assert out == "8"
+    env["GPU_MAX_HW_QUEUES"] = "4"
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         check=True).stdout.strip()
+    assert out == "4"
