@@ -304,8 +304,8 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
-              "(see the sharding section of DESIGN.md) and is not measured", file=sys.stderr)
+        print("bench.py: runs on one GPU only; sharding the sweep across GPUs is designed "
+              "but neither implemented nor measured", file=sys.stderr)
         sys.exit(2)
     import bricklib_amd as bk
     from bricklib_amd import stencils as S

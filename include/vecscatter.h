@@ -9,7 +9,7 @@
  * library's hand-written kernels (bk_stencil_from_file) and a whole sweep
  * over a list of bricks runs on the GPU through brick_sweep() in
  * brick-hip.h.  The per-brick `brick(...)` statement therefore becomes one
- * call after the loop that used to contain it (see INTEGRATION.md).
+ * call after the loop that used to contain it.
  */
 #ifndef BK_VECSCATTER_H
 #define BK_VECSCATTER_H

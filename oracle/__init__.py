@@ -1,8 +1,8 @@
 """ctypes wrapper of the oracle (oracle/oracle.c) and of oracle/_ref.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-the CPU-baseline leg of bench.py, always as the checker, never as the thing
-measured on the GPU or shipped.
+the CPU-baseline leg of bench.py, always as the reference the GPU results are
+compared with, never as the thing measured on the GPU or shipped.
 """
 from __future__ import annotations
 

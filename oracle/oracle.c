@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement of bricklib's brick-layout stencil path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker the parity tests, the
+ * TEST INFRASTRUCTURE ONLY.  This file is the reference the parity tests, the
  * smoke check and the CPU-baseline leg of the benchmark compare against.
  * Nothing in the product library calls, links or loads it.
  *
