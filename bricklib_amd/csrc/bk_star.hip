@@ -108,6 +108,27 @@ struct Faces {
     }
   }
 
+  // the values of planes 4H .. 4H+3 into tile planes 0 .. 3 (half tiles)
+  template <int H>
+  __device__ __forceinline__ void store_half(A *tile, int lane) const {
+    using TL = Tile<R, sizeof(A)>;
+    const int c = lane & 7, rr = (lane >> 3) % R, kq = (lane >> 3) / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      // radius 2 and 4: a value's half is fixed by q; radius 1: by kq
+      if constexpr (R > 1) {
+        if ((q * (8 / R)) / 4 != H) continue;
+      } else {
+        if ((kq >> 2) != H) continue;
+      }
+      const int k = q * (8 / R) + kq - 4 * H;
+      tile[TL::at(k, rr, R + c)] = jm[q];
+      tile[TL::at(k, 8 + R + rr, R + c)] = jp[q];
+      tile[TL::at(k, R + c, rr)] = im[q];
+      tile[TL::at(k, R + c, 8 + R + rr)] = ip[q];
+    }
+  }
+
   // into planes k0 .. k0+7 of the tile
   __device__ __forceinline__ void store(A *tile, int k0, int lane) const {
     using TL = Tile<R, sizeof(A)>;
@@ -137,10 +158,12 @@ struct StarBrick {
   // plane.  (The transposed assignment i = L >> 3, j = L & 7 removes the
   // two-way LDS read conflicts of radius 1 in FP64 but ran the 7-point
   // sweep 5% slower: lanes then address 64-byte-apart cells.)
-  __device__ __forceinline__ void load(const uint32_t *__restrict__ adj, const T *__restrict__ in,
-                                       size_t st, uint32_t b, A *tile, int lane) {
-    using TL = Tile<R, sizeof(A)>;
-    const int j = lane >> 3, i = lane & 7, e = lane;
+  // The two phases of load(): the brick's planes, k-faces and i/j faces
+  // into registers, then the planes and faces into this wave's LDS tile.
+  __device__ __forceinline__ void load_regs(const uint32_t *__restrict__ adj,
+                                            const T *__restrict__ in, size_t st, uint32_t b,
+                                            Faces<T, R> &f, int lane) {
+    const int e = lane;
     const uint32_t *ab = adj + (size_t)b * 27;
     const T *pc = in + (size_t)b * st;
     const T *pkm = in + (size_t)uniform(ab[4]) * st;
@@ -156,42 +179,64 @@ struct StarBrick {
       vkm[r] = widen(pkm[(8 - R + r) * 64 + e]);
       vkp[r] = widen(pkp[r * 64 + e]);
     }
-    Faces<T, R> f;
     f.load(pjm, pjp, pim, pip, lane);
+  }
+  __device__ __forceinline__ void stage(A *tile, const Faces<T, R> &f, int lane) const {
+    using TL = Tile<R, sizeof(A)>;
+    const int j = lane >> 3, i = lane & 7;
 #pragma unroll
     for (int k = 0; k < 8; ++k) tile[TL::at(k, R + j, R + i)] = v[k];
     f.store(tile, 0, lane);
   }
+  __device__ __forceinline__ void load(const uint32_t *__restrict__ adj, const T *__restrict__ in,
+                                       size_t st, uint32_t b, A *tile, int lane) {
+    Faces<T, R> f;
+    load_regs(adj, in, st, b, f, lane);
+    stage(tile, f, lane);
+  }
+  // planes 4H .. 4H+3 and their faces into a 4-plane tile
+  template <int H>
+  __device__ __forceinline__ void stage_half(A *tile, const Faces<T, R> &f, int lane) const {
+    using TL = Tile<R, sizeof(A)>;
+    const int j = lane >> 3, i = lane & 7;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) tile[TL::at(k, R + j, R + i)] = v[4 * H + k];
+    f.template store_half<H>(tile, lane);
+  }
 
-  // out: the output field, ost its step; b's 512 points go to brick b and
+  // radius 1 and 2 keep the brick's 8 output planes in registers for the
+  // image stores after the plane loop (3 more VGPRs for the 7-point FP64
+  // kernel); radius 4 stores its images plane by plane (keeping them would
+  // take it from 79 to 115 VGPRs)
+  static constexpr bool kImagesAfter = R <= 2;
+  A o[kImagesAfter ? 8 : 1];
+
+  // out: the output field, ost its step; planes K0 .. K0+NK-1 of b's points
+  // (their i/j taps at tile plane k - TK0) go to brick b and, for radius 4,
   // to its nimg ghost images imgs[0..nimg) (the fused refresh; nimg = 0
   // without it)
-  __device__ __forceinline__ void compute(const A (&c)[6 * R + 1], A theta, int cond_mode,
-                                          T *__restrict__ out, size_t ost, uint32_t b,
-                                          const A *tile, int lane,
-                                          const uint32_t *__restrict__ imgs = nullptr,
-                                          uint32_t nimg = 0) {
+  template <int K0, int NK, int TK0>
+  __device__ __forceinline__ void compute_planes(const A (&c)[6 * R + 1], A theta, int cond_mode,
+                                                 T *__restrict__ out, size_t ost, uint32_t b,
+                                                 const A *tile, int lane,
+                                                 const uint32_t *__restrict__ imgs,
+                                                 uint32_t nimg) {
     T *__restrict__ po = out + (size_t)b * ost;
     using TL = Tile<R, sizeof(A)>;
     const int j = lane >> 3, i = lane & 7, e = lane;
-    // radius 1 and 2 keep the brick's 8 output planes in registers for the
-    // image stores after the plane loop (3 more VGPRs for the 7-point FP64
-    // kernel); radius 4 stores its images plane by plane (keeping them would
-    // take it from 79 to 115 VGPRs)
-    constexpr bool kImagesAfter = R <= 2;
-    A o[kImagesAfter ? 8 : 1];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = K0; k < K0 + NK; ++k) {
       auto X = [&](int kk) -> A {
         return kk < 0 ? vkm[kk + R] : (kk >= 8 ? vkp[kk - 8] : v[kk]);
       };
+      const int t = k - TK0;
       A acc = c[0] * tapv<A, COND>(v[k]);
 #pragma unroll
       for (int r = 1; r <= R; ++r) {
-        acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[TL::at(k, R + j, R + i + r)]), acc);
-        acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[TL::at(k, R + j, R + i - r)]), acc);
-        acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[TL::at(k, R + j + r, R + i)]), acc);
-        acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[TL::at(k, R + j - r, R + i)]), acc);
+        acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[TL::at(t, R + j, R + i + r)]), acc);
+        acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[TL::at(t, R + j, R + i - r)]), acc);
+        acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[TL::at(t, R + j + r, R + i)]), acc);
+        acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[TL::at(t, R + j - r, R + i)]), acc);
         acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
         acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
       }
@@ -207,22 +252,48 @@ struct StarBrick {
           store_nt(&out[(size_t)uniform(imgs[q]) * ost + k * 64 + e], acc);
       }
     }
-    // the ghost images: each image's id read once, its 8 planes stored in
-    // a row (7-point FP64 fused sweep 475.7 -> 467.0 us per step)
+  }
+
+  // the ghost images of radius 1 and 2: each image's id read once, its 8
+  // planes stored in a row (7-point FP64 fused sweep 475.7 -> 467.0 us per
+  // step against storing them plane by plane)
+  __device__ __forceinline__ void store_images(T *__restrict__ out, size_t ost, int lane,
+                                               const uint32_t *__restrict__ imgs,
+                                               uint32_t nimg) const {
     if constexpr (kImagesAfter) {
       for (uint32_t q = 0; q < nimg; ++q) {
-        T *__restrict__ pq = out + (size_t)uniform(imgs[q]) * ost + e;
+        T *__restrict__ pq = out + (size_t)uniform(imgs[q]) * ost + lane;
 #pragma unroll
         for (int k = 0; k < 8; ++k) store_nt(&pq[k * 64], o[k]);
       }
     }
   }
+
+  // all 8 planes from a full tile
+  __device__ __forceinline__ void compute(const A (&c)[6 * R + 1], A theta, int cond_mode,
+                                          T *__restrict__ out, size_t ost, uint32_t b,
+                                          const A *tile, int lane,
+                                          const uint32_t *__restrict__ imgs = nullptr,
+                                          uint32_t nimg = 0) {
+    compute_planes<0, 8, 0>(c, theta, cond_mode, out, ost, b, tile, lane, imgs, nimg);
+    store_images(out, ost, lane, imgs, nimg);
+  }
 };
+
+// FP64 radius 1 and 2 stage the LDS tile in two halves of four planes (all
+// eight planes are in registers from the start; only the staging is split):
+// a full FP64 tile held the 7-point star at 6 waves per SIMD by LDS (25.6 KB
+// per workgroup) and the 13-point at 4 (36.9 KB).
+template <class T, int R>
+__host__ __device__ constexpr bool lds_half_tiles() {
+  return sizeof(typename acc_of<T>::type) == 8 && R <= 2;
+}
 
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[4][8 * Tile<R, sizeof(A)>::PL];
+  constexpr bool kHalf = lds_half_tiles<T, R>();
+  __shared__ A tile[4][(kHalf ? 4 : 8) * Tile<R, sizeof(A)>::PL];
 
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -231,23 +302,50 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
 
   StarBrick<T, R, COND> sb;
   uint32_t b = 0;
-  if (active) {
-    b = uniform(a.list[slot]);
-    sb.load(a.adj, a.in, a.in_step, b, tile[w], lane);
-  }
-  __syncthreads();
-  if (!active) return;
   A c[6 * R + 1];
-#pragma unroll
-  for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
   const uint32_t *imgs = nullptr;
   uint32_t nimg = 0;
-  if (a.img_off) {
-    const uint32_t i0 = uniform(a.img_off[b]);
-    nimg = uniform(a.img_off[b + 1]) - i0;
-    imgs = a.img + i0;
+  auto prepare = [&]() {
+#pragma unroll
+    for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
+    if (a.img_off) {
+      const uint32_t i0 = uniform(a.img_off[b]);
+      nimg = uniform(a.img_off[b + 1]) - i0;
+      imgs = a.img + i0;
+    }
+  };
+  if constexpr (!kHalf) {
+    if (active) {
+      b = uniform(a.list[slot]);
+      sb.load(a.adj, a.in, a.in_step, b, tile[w], lane);
+    }
+    __syncthreads();
+    if (!active) return;
+    prepare();
+    sb.compute(c, a.theta, a.cond_mode, a.out, a.out_step, b, tile[w], lane, imgs, nimg);
+  } else {
+    // every wave passes all three barriers, active or not
+    Faces<T, R> f;
+    if (active) {
+      b = uniform(a.list[slot]);
+      sb.load_regs(a.adj, a.in, a.in_step, b, f, lane);
+      sb.template stage_half<0>(tile[w], f, lane);
+    }
+    __syncthreads();
+    if (active) {
+      prepare();
+      sb.template compute_planes<0, 4, 0>(c, a.theta, a.cond_mode, a.out, a.out_step, b, tile[w],
+                                          lane, imgs, nimg);
+    }
+    __syncthreads();  // this wave's reads of the first half are done
+    if (active) sb.template stage_half<1>(tile[w], f, lane);
+    __syncthreads();
+    if (active) {
+      sb.template compute_planes<4, 4, 4>(c, a.theta, a.cond_mode, a.out, a.out_step, b,
+                                          tile[w], lane, imgs, nimg);
+      sb.store_images(a.out, a.out_step, lane, imgs, nimg);
+    }
   }
-  sb.compute(c, a.theta, a.cond_mode, a.out, a.out_step, b, tile[w], lane, imgs, nimg);
 }
 
 
