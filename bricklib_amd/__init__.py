@@ -10,12 +10,19 @@ import os as _os
 
 # HIP gives a process GPU_MAX_HW_QUEUES hardware queues (4 unless set) and
 # shares them among its streams beyond that count.  The serving loop runs
-# two streams concurrently; with four queues, whether they land on separate
-# queues depends on how many streams the process already holds (measured:
-# one stream more, and the two lanes ran one after the other, p50 latency
-# 10 -> 13.5 ms).  Eight queues keep them apart.  Takes effect only when
-# HIP is initialised after this import; a value set by the user is kept.
-_os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# two streams concurrently (plus a capture stream per batch); with four
+# queues, whether the two land on separate queues depends on how many
+# streams the process already holds (measured: one stream more, and the
+# two lanes ran one after the other, p50 latency 10 -> 13.5 ms).  Eight
+# queues keep them apart, so a value below 8 (or none) is raised to 8; a
+# higher one is kept.  Takes effect only when HIP is initialised after this
+# import.
+try:
+    _queues = int(_os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _queues = 0
+if _queues < 8:
+    _os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 from ._lib import (BK_BF16, BK_COND7, BK_COND_REFERENCE, BK_COND_THRESHOLD, BK_CUBE125,
                    BK_F32, BK_F64, BK_STAR7, BK_STAR13, BK_STAR25, BrickError, load)

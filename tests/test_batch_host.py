@@ -48,9 +48,10 @@ def test_serving_lanes_pair_cubes_with_the_other_dtypes_stars():
     assert all(lane_of(k, d, 1) == 0 for k in range(5) for d in (BK_F64, BK_F32, BK_BF16))
 
 
-def test_import_raises_the_hardware_queue_default_but_keeps_a_users_value():
+def test_import_raises_the_hardware_queue_count_to_eight():
     """The serving lanes need their streams on separate hardware queues;
-    importing the package sets GPU_MAX_HW_QUEUES=8 unless it is set."""
+    importing the package raises GPU_MAX_HW_QUEUES to 8 when it is unset or
+    lower, and keeps a higher value."""
     import os
     import subprocess
     import sys
@@ -61,7 +62,8 @@ def test_import_raises_the_hardware_queue_default_but_keeps_a_users_value():
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                          check=True).stdout.strip()
     assert out == "8"
-    env["GPU_MAX_HW_QUEUES"] = "4"
-    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
-                         check=True).stdout.strip()
-    assert out == "4"
+    for given, want in (("4", "8"), ("16", "16"), ("bogus", "8")):
+        env["GPU_MAX_HW_QUEUES"] = given
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                             text=True, check=True).stdout.strip()
+        assert out == want, (given, out)
