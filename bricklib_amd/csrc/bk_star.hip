@@ -349,6 +349,14 @@ __global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND>
 }
 
 
+// The FP64 radius-4 column kernel stages its LDS tile in two halves of four
+// planes: a full 16 KB tile per one-wave workgroup held it at 2.5 waves per
+// SIMD (10 workgroups per CU), below the 3 its registers allow.
+template <class T, int R>
+__host__ __device__ constexpr bool lds_half_columns() {
+  return sizeof(typename acc_of<T>::type) == 8 && R == 4;
+}
+
 // Column-marching schedule: one wave per plan unit, a run of up to kcol
 // bricks b, adj[b][22], ... along k.  While brick l is computed, brick l+1
 // is already being loaded into registers; its plane 0 is the k+1 face of
@@ -395,32 +403,56 @@ __device__ __forceinline__ void column_unit(const uint32_t *__restrict__ adj,
 #pragma unroll
       for (int r = 0; r < R; ++r) nxt[r] = widen(pu[r * 64 + lane]);
     }
+    const bool store_out = (mk[l] & 1u) != 0;
+    T *po = out + (size_t)b * ost;
+    // output plane k of this brick, its i/j taps at tile plane t
+    auto plane = [&](int k, int t) {
+      auto X = [&](int kk) -> A {
+        return kk < 0 ? km[kk + R] : (kk >= 8 ? nxt[kk - 8] : cur[kk]);
+      };
+      A acc = c[0] * tapv<A, COND>(cur[k]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) tile[TL::at(k, R + j, R + i)] = cur[k];
-    f.store(tile, 0, lane);
-    __syncthreads();
-    if (mk[l] & 1u) {
-      T *po = out + (size_t)b * ost;
+      for (int r = 1; r <= R; ++r) {
+        acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[TL::at(t, R + j, R + i + r)]), acc);
+        acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[TL::at(t, R + j, R + i - r)]), acc);
+        acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[TL::at(t, R + j + r, R + i)]), acc);
+        acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[TL::at(t, R + j - r, R + i)]), acc);
+        acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
+        acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
+      }
+      if constexpr (COND) {
+        const bool sel = cond_mode == 0 ? (acc > A(0)) : (cur[k] > theta);
+        acc = sel ? acc : -acc;
+      }
+      store_nt(&po[k * 64 + lane], acc);
+    };
+    if constexpr (!lds_half_columns<T, R>()) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        auto X = [&](int kk) -> A {
-          return kk < 0 ? km[kk + R] : (kk >= 8 ? nxt[kk - 8] : cur[kk]);
-        };
-        A acc = c[0] * tapv<A, COND>(cur[k]);
+      for (int k = 0; k < 8; ++k) tile[TL::at(k, R + j, R + i)] = cur[k];
+      f.store(tile, 0, lane);
+      __syncthreads();
+      if (store_out) {
 #pragma unroll
-        for (int r = 1; r <= R; ++r) {
-          acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[TL::at(k, R + j, R + i + r)]), acc);
-          acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[TL::at(k, R + j, R + i - r)]), acc);
-          acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[TL::at(k, R + j + r, R + i)]), acc);
-          acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[TL::at(k, R + j - r, R + i)]), acc);
-          acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
-          acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
-        }
-        if constexpr (COND) {
-          const bool sel = cond_mode == 0 ? (acc > A(0)) : (cur[k] > theta);
-          acc = sel ? acc : -acc;
-        }
-        store_nt(&po[k * 64 + lane], acc);
+        for (int k = 0; k < 8; ++k) plane(k, k);
+      }
+    } else {
+      // the tile in two halves of four planes
+#pragma unroll
+      for (int k = 0; k < 4; ++k) tile[TL::at(k, R + j, R + i)] = cur[k];
+      f.template store_half<0>(tile, lane);
+      __syncthreads();
+      if (store_out) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) plane(k, k);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) tile[TL::at(k, R + j, R + i)] = cur[4 + k];
+      f.template store_half<1>(tile, lane);
+      __syncthreads();
+      if (store_out) {
+#pragma unroll
+        for (int k = 4; k < 8; ++k) plane(k, k - 4);
       }
     }
 #pragma unroll
@@ -437,7 +469,7 @@ __device__ __forceinline__ void column_unit(const uint32_t *__restrict__ adj,
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(64) void star_sweep_columns(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[8 * Tile<R, sizeof(A)>::PL];
+  __shared__ A tile[(lds_half_columns<T, R>() ? 4 : 8) * Tile<R, sizeof(A)>::PL];
   const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
   if (u >= a.nunits) return;
   A c[6 * R + 1];
@@ -455,7 +487,7 @@ template <class T, int R, bool COND>
 __global__ __launch_bounds__(64) void star_sweep_columns_batched(const BatchTable *tab_,
                                                                  uint32_t tick) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[8 * Tile<R, sizeof(A)>::PL];
+  __shared__ A tile[(lds_half_columns<T, R>() ? 4 : 8) * Tile<R, sizeof(A)>::PL];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
