@@ -280,13 +280,13 @@ struct StarBrick {
   }
 };
 
-// FP64 radius 1 and 2 stage the LDS tile in two halves of four planes (all
+// FP64 stages the per-brick LDS tile in two halves of four planes (all
 // eight planes are in registers from the start; only the staging is split):
 // a full FP64 tile held the 7-point star at 6 waves per SIMD by LDS (25.6 KB
-// per workgroup) and the 13-point at 4 (36.9 KB).
+// per workgroup), the 13-point at 4 (36.9 KB) and the 25-point at 2 (64 KB).
 template <class T, int R>
 __host__ __device__ constexpr bool lds_half_tiles() {
-  return sizeof(typename acc_of<T>::type) == 8 && R <= 2;
+  return sizeof(typename acc_of<T>::type) == 8;
 }
 
 template <class T, int R, bool COND>
@@ -641,7 +641,7 @@ hipError_t launch_cols_dt(const SweepLaunch &L, const PlanView &P, hipStream_t s
 template <class T, int R, bool COND>
 __global__ __launch_bounds__(256) void star_sweep_batched(const BatchTable *tab_, uint32_t tick) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[4][8 * Tile<R, sizeof(A)>::PL];
+  __shared__ A tile[4][(lds_half_tiles<T, R>() ? 4 : 8) * Tile<R, sizeof(A)>::PL];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
@@ -661,23 +661,58 @@ __global__ __launch_bounds__(256) void star_sweep_batched(const BatchTable *tab_
     }
     StarBrick<T, R, COND> sb;
     uint32_t b = 0;
-    if (active) {
-      const auto &sg = tab->seg[cur.s];
-      const bool odd = ((sg.parity + tick) & 1u) != 0;
-      const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
-      b = sg.items[item - cur.lo];
-      sb.load(sg.adj, in, sg.step, b, tile[w], lane);
-    }
-    __syncthreads();
-    if (active) {
-      const auto &sg = tab->seg[cur.s];
-      const bool odd = ((sg.parity + tick) & 1u) != 0;
-      T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);
-      const kconst_ptr<A> cp = (kconst_ptr<A>)sg.coeff;
-      A c[6 * R + 1];
+    if constexpr (!lds_half_tiles<T, R>()) {
+      if (active) {
+        const auto &sg = tab->seg[cur.s];
+        const bool odd = ((sg.parity + tick) & 1u) != 0;
+        const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
+        b = sg.items[item - cur.lo];
+        sb.load(sg.adj, in, sg.step, b, tile[w], lane);
+      }
+      __syncthreads();
+      if (active) {
+        const auto &sg = tab->seg[cur.s];
+        const bool odd = ((sg.parity + tick) & 1u) != 0;
+        T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);
+        const kconst_ptr<A> cp = (kconst_ptr<A>)sg.coeff;
+        A c[6 * R + 1];
 #pragma unroll
-      for (int q = 0; q < 6 * R + 1; ++q) c[q] = cp[q];
-      sb.compute(c, (A)sg.theta, sg.cond_mode, out, sg.step, b, tile[w], lane);
+        for (int q = 0; q < 6 * R + 1; ++q) c[q] = cp[q];
+        sb.compute(c, (A)sg.theta, sg.cond_mode, out, sg.step, b, tile[w], lane);
+      }
+    } else {
+      // the tile in two halves of four planes; every wave passes all the
+      // barriers, active or not
+      Faces<T, R> f;
+      if (active) {
+        const auto &sg = tab->seg[cur.s];
+        const bool odd = ((sg.parity + tick) & 1u) != 0;
+        const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
+        b = sg.items[item - cur.lo];
+        sb.load_regs(sg.adj, in, sg.step, b, f, lane);
+        sb.template stage_half<0>(tile[w], f, lane);
+      }
+      __syncthreads();
+      A c[6 * R + 1];
+      T *out = nullptr;
+      if (active) {
+        const auto &sg = tab->seg[cur.s];
+        const bool odd = ((sg.parity + tick) & 1u) != 0;
+        out = static_cast<T *>(odd ? sg.f0 : sg.f1);
+        const kconst_ptr<A> cp = (kconst_ptr<A>)sg.coeff;
+#pragma unroll
+        for (int q = 0; q < 6 * R + 1; ++q) c[q] = cp[q];
+        sb.template compute_planes<0, 4, 0>(c, (A)sg.theta, sg.cond_mode, out, sg.step, b,
+                                            tile[w], lane, nullptr, 0u);
+      }
+      __syncthreads();
+      if (active) sb.template stage_half<1>(tile[w], f, lane);
+      __syncthreads();
+      if (active) {
+        const auto &sg = tab->seg[cur.s];
+        sb.template compute_planes<4, 4, 4>(c, (A)sg.theta, sg.cond_mode, out, sg.step, b,
+                                            tile[w], lane, nullptr, 0u);
+      }
     }
     __syncthreads();
   }
