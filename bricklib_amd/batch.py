@@ -47,13 +47,13 @@ def _record_dtype():
 REQ_DTYPE = _record_dtype()
 
 
-# star stencils whose batched sweeps march k-columns (the column schedule)
-# instead of one brick per wave.  None by default: the batched column kernel
-# is a persistent grid of one-wave workgroups with a static share of units
-# each, and with 2.5 such waves per SIMD (the radius-4 FP64 tile's LDS
-# limit) the SIMDs holding three finish last -- on 60 grids of 32..256
-# cells, 25-point FP64 ran 156 GStencil/s by columns against 184 one brick
-# per wave; FP32 362 against 368.
+# (stencil, dtype) pairs whose batched sweeps march k-columns (the column
+# schedule) instead of one brick per wave.  None by default.  A batch of 60
+# FP64 25-point grids of 32..256 cells alone runs faster by columns (209.7
+# GStencil/s against 175.4; FP32 362 against 384), but in the serving mix
+# the column units -- up to eight bricks swept in turn by one wave -- make
+# ticks with little 25-point work last longer: the saturating run took
+# 0.409 s against 0.365 and the Poisson stream's p50 rose 10.5 -> 13.3 ms.
 BATCH_COLUMN_STENCILS: frozenset = frozenset()
 
 
@@ -88,7 +88,7 @@ class Shape:
         atomics cost every short launch several microseconds)."""
         if stencil_kind == BK_CUBE125:
             return self.grid.plan("quad", 4)
-        if stencil_kind in BATCH_COLUMN_STENCILS:
+        if (stencil_kind, dtype) in BATCH_COLUMN_STENCILS:
             return self.columns
         return None
 
