@@ -246,7 +246,10 @@ def serve_bench(bk, S, args):
 
     # real-time Poisson stream
     reqs = SV.workload(args.serve_requests, rate=args.serve_rate, seed=2024)
-    srv = SV.Server(hbm_budget=200e9, chunk=8)
+    # chunks of 16 ticks: at this load a chunk's GPU time is short and the
+    # host's round trip per chunk adds to every request's latency (p50 10.5
+    # ms with chunks of 8, 8.5-9.0 with 16 or 32, 11 with 64)
+    srv = SV.Server(hbm_budget=200e9, chunk=16)
     srv.prewarm(reqs)
     sample = set(r.rid for r in reqs[::max(1, len(reqs) // 12)])
     st = srv.serve(reqs, realtime=True, keep_results=sample)

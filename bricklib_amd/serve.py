@@ -89,9 +89,9 @@ def lane_of(kind: int, dtype: int, lanes: int) -> int:
 
 
 class Server:
-    """Continuous-batching loop over one Batch."""
+    """Continuous-batching loop over `lanes` batches (one stream each)."""
 
-    def __init__(self, hbm_budget: float = 200e9, chunk: int = 8, max_active: int = 2048,
+    def __init__(self, hbm_budget: float = 200e9, chunk: int = 16, max_active: int = 2048,
                  use_graph: bool = True, ghost: int = 0, slab_bytes: int = 1 << 30,
                  lanes: int = 2):
         if lanes not in (1, 2):
