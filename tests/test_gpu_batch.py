@@ -41,12 +41,15 @@ def alone(gpu, r: Request, ghost: int = 1) -> np.ndarray:
     return res.storage.to_host()[g.interior]
 
 
-@pytest.mark.parametrize("chunk,ghost,lanes", [(1, 0, 2), (3, 0, 2), (8, 0, 2), (3, 1, 2),
-                                               (8, 1, 2), (3, 0, 1), (8, 1, 1)])
-def test_mix_matches_alone(gpu, chunk, ghost, lanes):
+@pytest.mark.parametrize("chunk,ghost,lanes,pipeline", [
+    (1, 0, 2, True), (3, 0, 2, True), (8, 0, 2, True), (3, 1, 2, True), (8, 1, 2, True),
+    (3, 0, 1, True), (8, 1, 1, True), (3, 0, 2, False), (8, 1, 1, False)])
+def test_mix_matches_alone(gpu, chunk, ghost, lanes, pipeline):
     """Every request of a mixed batch -- on one stream, or split over two
-    lanes that run concurrently -- gives its single-grid result bit for bit."""
-    srv = serve.Server(chunk=chunk, ghost=ghost, lanes=lanes)
+    lanes that run concurrently, with the next chunk queued before the
+    current one is waited for or not -- gives its single-grid result bit
+    for bit."""
+    srv = serve.Server(chunk=chunk, ghost=ghost, lanes=lanes, pipeline=pipeline)
     reqs = make_mix()
     if lanes == 2:
         assert {srv.lane(r) for r in reqs} == {0, 1}
@@ -54,7 +57,7 @@ def test_mix_matches_alone(gpu, chunk, ghost, lanes):
     assert len(stats.completed) == len(reqs)
     for r in reqs:
         print(f"[case] rid={r.rid} {r.nb} kind={r.stencil.kind} dtype={r.dtype} "
-              f"steps={r.steps} ghost={ghost} lanes={lanes}", flush=True)
+              f"steps={r.steps} ghost={ghost} lanes={lanes} pipeline={pipeline}", flush=True)
         np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
 
 
