@@ -90,6 +90,7 @@ SIGNATURES = {
     "bk_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "bk_set_device": (C.c_int, [C.c_int]),
     "bk_device_sync": (C.c_int, []),
+    "bk_pointer_is_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "bk_storage_alloc": (C.c_int, [C.c_int, C.c_long, C.c_size_t, C.POINTER(vp)]),
     "bk_free": (C.c_int, [vp]),
     "bk_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),

@@ -17,6 +17,7 @@
 #include <utility>
 
 #include "bk_api.h"
+#include "bk_graphs.h"
 #include "bk_internal.h"
 
 namespace bk {
@@ -48,7 +49,10 @@ struct bk_batch_s {
   uint32_t ref_mask = 0;    // non-empty refresh tables
   hipEvent_t staged = nullptr;
   hipStream_t capture = nullptr;
-  std::map<std::pair<uint32_t, uint64_t>, hipGraphExec_t> graphs;
+  // captured runs keyed by (tick count, refresh and sweep group masks); an
+  // exec is destroyed only once its last launch has completed, so a caller
+  // may queue the next run while the previous one is still on the device
+  bk::ExecCache<std::pair<uint32_t, uint64_t>> graphs{64};
   std::mutex mu;
 };
 
@@ -98,7 +102,7 @@ extern "C" {
 
 int bk_batch_destroy(bk_batch b) {
   if (!b) return BK_OK;
-  for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  (void)b->graphs.clear();
   for (int g = 0; g < kGroups; ++g) {
     (void)hipFree(b->d_sweep[g]);
     (void)hipHostFree(b->h_sweep[g]);
@@ -288,12 +292,7 @@ int bk_batch_run(bk_batch b, uint32_t nticks, int use_graph, bk_stream stream) {
   }
   const std::pair<uint32_t, uint64_t> key{nticks,
                                           ((uint64_t)b->ref_mask << 32) | (uint64_t)b->sweep_mask};
-  auto it = b->graphs.find(key);
-  if (it == b->graphs.end()) {
-    if (b->graphs.size() >= 64) {
-      for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-      b->graphs.clear();
-    }
+  if (!b->graphs.find(key)) {
     BKB_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueue_run(b, nticks, b->capture);
     hipGraph_t g = nullptr;
@@ -307,9 +306,9 @@ int bk_batch_run(bk_batch b, uint32_t nticks, int use_graph, bk_stream stream) {
     e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
-    it = b->graphs.emplace(key, ge).first;
+    BKB_HIP(b->graphs.insert(key, ge));
   }
-  BKB_HIP(hipGraphLaunch(it->second, s));
+  BKB_HIP(b->graphs.launch(key, s));
   return BK_OK;
 }
 

@@ -42,6 +42,7 @@ struct PlanView {
   const uint32_t *flat;
   const uint32_t *start;
   uint32_t nbricks;  // bricks of the adjacency the plan was built on
+  uint64_t id;       // unique per plan (graph-cache key), never 0
 };
 const PlanView plan_view(const bk_plan_s *p);
 

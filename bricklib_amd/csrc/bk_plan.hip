@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -29,6 +30,7 @@ int plan_fail(int code, const char *msg);
 int plan_build(const uint32_t *h_adj, uint32_t nbricks, const uint32_t *h_list, uint32_t nlist,
                int shape, uint32_t kcol, uint32_t *h_anchor, uint32_t *h_len, uint32_t *h_mask,
                uint32_t *h_flat, uint32_t *h_start, uint32_t *nunits, uint32_t *ncovered);
+void graphs_forget_plan(uint64_t plan_id);
 }
 
 struct bk_plan_s {
@@ -43,7 +45,12 @@ struct bk_plan_s {
   uint32_t *d_start = nullptr;   // [nunits] offset of each unit's bricks in d_flat
   uint32_t nbricks = 0;          // bricks of the adjacency the plan was built on
   int device = 0;
+  uint64_t id = 0;               // unique per plan for the process's lifetime (never 0)
 };
+
+namespace {
+std::atomic<uint64_t> g_next_plan_id{1};
+}
 
 namespace bk {
 
@@ -59,6 +66,7 @@ const PlanView plan_view(const bk_plan_s *p) {
   v.flat = p->d_flat;
   v.start = p->d_start;
   v.nbricks = p->nbricks;
+  v.id = p->id;
   return v;
 }
 
@@ -287,6 +295,7 @@ int bk_plan_create(const uint32_t *h_adj, uint32_t nbricks, const uint32_t *h_li
   p->nunits = nu;
   p->ncovered = nc;
   p->nbricks = nbricks;
+  p->id = g_next_plan_id.fetch_add(1);
   hipGetDevice(&p->device);
   auto up = [](uint32_t **dst, const std::vector<uint32_t> &v) -> hipError_t {
     if (v.empty()) return hipSuccess;
@@ -310,6 +319,8 @@ int bk_plan_create(const uint32_t *h_adj, uint32_t nbricks, const uint32_t *h_li
 
 int bk_plan_destroy(bk_plan p) {
   if (!p) return BK_OK;
+  // cached graphs captured the plan's device arrays: drop them first
+  bk::graphs_forget_plan(p->id);
   hipFree(p->d_anchor);
   hipFree(p->d_len);
   hipFree(p->d_mask);

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "bk_api.h"
+#include "bk_graphs.h"
 #include "bk_internal.h"
 
 namespace {
@@ -74,10 +75,13 @@ int get_scratch(double **p) {
   return BK_OK;
 }
 
-// Graph cache for bk_sweeps(use_graph): key = every argument byte.
+// Graph cache for bk_sweeps(use_graph): key = every argument byte, with
+// plans keyed by their ids (a plan's host handle can be reused by a later
+// plan once freed; its id cannot).  At most 64 graphs, evicted least
+// recently used once their last launch has completed.
 struct GraphCache {
   std::mutex mu;
-  std::map<std::string, hipGraphExec_t> exec;
+  bk::ExecCache<std::string> exec{64};
   std::map<int, hipStream_t> capture_stream;
 };
 GraphCache &graphs() {
@@ -85,7 +89,11 @@ GraphCache &graphs() {
   return g;
 }
 
-int check_desc(const bk_sweep_desc *d) {
+uint64_t plan_id(const bk_plan_s *p) { return p ? bk::plan_view(p).id : 0; }
+
+// pingpong: the sequence also reads d_out (with out_step) as an input, so
+// a bound on input addressing applies to both fields.
+int check_desc(const bk_sweep_desc *d, bool pingpong = false) {
   if (!d) return fail(BK_ERR_ARG, "bk_sweep: null descriptor");
   const int nc = bk_stencil_ncoeff(d->stencil);
   if (nc == 0) return fail(BK_ERR_ARG, "bk_sweep: unknown stencil %d", d->stencil);
@@ -103,7 +111,8 @@ int check_desc(const bk_sweep_desc *d) {
     return fail(BK_ERR_ARG, "bk_sweep: step smaller than a brick (512 elements)");
   if (d->plan && d->stencil == BK_CUBE125) {
     const bk::PlanView P = bk::plan_view(d->plan);
-    if (P.shape == BK_PLAN_QUAD && !bk::quad_offsets_fit(P.nbricks, d->in_step))
+    const size_t in_step = pingpong ? std::max(d->in_step, d->out_step) : d->in_step;
+    if (P.shape == BK_PLAN_QUAD && !bk::quad_offsets_fit(P.nbricks, in_step))
       return fail(BK_ERR_UNSUPPORTED,
                   "bk_sweep: the quad schedule addresses at most 2^32 input elements "
                   "(nbricks * step); use a smaller field or no plan");
@@ -201,6 +210,12 @@ void grid_dims(const long nb[3], long S[3]) {
 
 namespace bk {
 int plan_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+
+void graphs_forget_plan(uint64_t plan_id) {
+  auto &gc = graphs();
+  std::lock_guard<std::mutex> lock(gc.mu);
+  (void)gc.exec.forget(plan_id);
+}
 }  // namespace bk
 
 extern "C" {
@@ -250,6 +265,20 @@ int bk_set_device(int dev) {
 
 int bk_device_sync(void) {
   BK_HIP(hipDeviceSynchronize());
+  return BK_OK;
+}
+
+int bk_pointer_is_device(const void *p, int *is_device) {
+  if (!is_device) return fail(BK_ERR_ARG, "bk_pointer_is_device: null out pointer");
+  *is_device = 0;
+  if (!p) return BK_OK;
+  hipPointerAttribute_t a;
+  std::memset(&a, 0, sizeof(a));
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // plain host memory: not an error for the caller
+    return BK_OK;
+  }
+  *is_device = (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) ? 1 : 0;
   return BK_OK;
 }
 
@@ -599,7 +628,8 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
   put(&d->in_step, sizeof(d->in_step));
   put(&d->d_out, sizeof(d->d_out));
   put(&d->out_step, sizeof(d->out_step));
-  put(&d->plan, sizeof(d->plan));
+  const uint64_t pid = plan_id(d->plan), apid = plan_id(all_plan);
+  put(&pid, sizeof(pid));
   put(d->h_coeff, sizeof(double) * (size_t)bk_stencil_ncoeff(d->stencil));
   put(&d->theta, sizeof(d->theta));
   put(&d->cond_mode, sizeof(d->cond_mode));
@@ -609,17 +639,12 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
   put(&ghost_iters, sizeof(ghost_iters));
   put(&d_all, sizeof(d_all));
   put(&nall, sizeof(nall));
-  put(&all_plan, sizeof(all_plan));
+  put(&apid, sizeof(apid));
   put(&img_off, sizeof(img_off));
   put(&img, sizeof(img));
   auto &gc = graphs();
   std::lock_guard<std::mutex> lock(gc.mu);
-  auto it = gc.exec.find(key);
-  if (it == gc.exec.end()) {
-    if (gc.exec.size() >= 64) {
-      for (auto &kv : gc.exec) hipGraphExecDestroy(kv.second);
-      gc.exec.clear();
-    }
+  if (!gc.exec.find(key)) {
     hipStream_t cs;
     auto cit = gc.capture_stream.find(dev);
     if (cit == gc.capture_stream.end()) {
@@ -642,9 +667,9 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
     e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-    it = gc.exec.emplace(key, ge).first;
+    BK_HIP(gc.exec.insert(key, ge, pid, apid));
   }
-  BK_HIP(hipGraphLaunch(it->second, s));
+  BK_HIP(gc.exec.launch(key, s));
   return BK_OK;
 }
 
@@ -652,7 +677,7 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
 
 int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, int nsweeps,
               int use_graph, bk_stream s) {
-  int rc = check_desc(d);
+  int rc = check_desc(d, nsweeps > 1);
   if (rc != BK_OK) return rc;
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps: null pair list");
@@ -663,7 +688,7 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs, 
 int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off, const uint32_t *d_img,
                     const uint32_t *d_pairs, uint32_t npairs, int nsweeps, int use_graph,
                     bk_stream s) {
-  int rc = check_desc(d);
+  int rc = check_desc(d, nsweeps > 1);
   if (rc != BK_OK) return rc;
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps_fused: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps_fused: null pair list");
@@ -679,7 +704,7 @@ int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off, const uin
 int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
                    bk_plan all_plan, const uint32_t *d_pairs, uint32_t npairs, int nsweeps,
                    int ghost_iters, int use_graph, bk_stream s) {
-  int rc = check_desc(d);
+  int rc = check_desc(d, nsweeps > 1);
   if (rc != BK_OK) return rc;
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps_deep: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps_deep: null pair list");
@@ -693,7 +718,7 @@ int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
   all.d_list = d_all;
   all.nlist = nall;
   all.plan = all_plan;
-  rc = check_desc(&all);
+  rc = check_desc(&all, nsweeps > 1);
   if (rc != BK_OK) return rc;
   return sweeps_impl(d, d_pairs, npairs, nsweeps, ghost_iters, d_all, nall, all_plan, nullptr,
                      nullptr, use_graph, S(s));
@@ -702,8 +727,7 @@ int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
 int bk_graph_cache_clear(void) {
   auto &gc = graphs();
   std::lock_guard<std::mutex> lock(gc.mu);
-  for (auto &kv : gc.exec) hipGraphExecDestroy(kv.second);
-  gc.exec.clear();
+  BK_HIP(gc.exec.clear());
   return BK_OK;
 }
 

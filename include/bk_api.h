@@ -90,6 +90,11 @@ int bk_dtype_size(int dtype);
 int bk_device_count(int *n);
 int bk_set_device(int dev);
 int bk_device_sync(void);
+/* *is_device = 1 when p is device (or managed) memory a kernel may read,
+ * 0 for host memory.  No counterpart in the reference (its accessor kernels
+ * simply take what they are given); the C++ headers use it to refuse host
+ * pointers before a launch rather than fault the GPU. */
+int bk_pointer_is_device(const void *p, int *is_device);
 
 /* ---- memory: device-resident BrickStorage ----------------------------- */
 /* BrickStorage::allocate(chunks, step) (brick.h:68-75) with the data in HBM:

@@ -31,7 +31,7 @@
 #define ALIGN 2048
 #endif
 
-#if defined(__HIPCC__) || defined(__CUDACC__)
+#if defined(__HIPCC__)
 #define FORCUDA __host__ __device__
 #else
 #define FORCUDA
@@ -68,7 +68,7 @@ struct BrickStorage {
     b.step = step;
     size_t bytes = (size_t)chunks * step * sizeof(bElem);
     bytes = (bytes + ALIGN - 1) / ALIGN * ALIGN;
-    b.dat = std::shared_ptr<bElem>(static_cast<bElem *>(std::aligned_alloc(ALIGN, bytes)),
+    b.dat = std::shared_ptr<bElem>(static_cast<bElem *>(::aligned_alloc(ALIGN, bytes)),
                                    [](bElem *p) { std::free(p); });
     return b;
   }
@@ -205,6 +205,12 @@ struct Brick<Dim<BDims...>, Dim<Folds...>> {
   size_t step;
   bElem *dat;
   BrickStorage bStorage;
+
+  /// Extent of dimension d (outermost first).
+  static constexpr unsigned extent(unsigned d) {
+    constexpr unsigned D[NDIMS] = {BDims...};
+    return D[d];
+  }
 
   Brick(myBrickInfo *bInfo, const BrickStorage &brickStorage, unsigned offset) : bInfo(bInfo) {
     bStorage = brickStorage;

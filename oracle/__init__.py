@@ -121,25 +121,38 @@ def ghost_pairs(nb):
     return out
 
 
-class Grid:
-    """Host twin of bricklib_amd.PeriodicGrid: (nb+2)^3 bricks, one ghost
-    layer, numbering/adjacency of init_grid."""
+class Layout:
+    """The oracle over any brick numbering: an adjacency adj[nbricks][27],
+    the interior list a sweep covers and the (dst, src) ghost pairs a
+    refresh copies -- e.g. the BrickDecomp layout."""
 
-    def __init__(self, nb):
-        self.nb = tuple(int(x) for x in nb)
-        self.padded = tuple(x + 2 for x in self.nb)
-        self.grid, self.adj = init_grid(self.padded)
-        self.interior = interior_list(self.nb)
-        self.pairs = ghost_pairs(self.nb)
+    def __init__(self, adj, interior, pairs):
+        self.adj = np.ascontiguousarray(adj, np.uint32).reshape(-1, 27)
+        self.interior = np.ascontiguousarray(interior, np.uint32)
+        self.pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1)
         self.nbricks = self.adj.shape[0]
+
+    def deep_sweeps(self, stencil, coeff, a, b, n, ghost_iters, dtype=F64, mode=MODE_FMA,
+                    theta=0.0, cond_mode=0):
+        """The communication-avoiding loop of weak/main.cpp:26-36,203-211:
+        the ghost bricks of the input are refreshed before sweeps
+        0, g, 2g, ...; the last sweep of each group of g (and of the
+        sequence) covers the interior (skip = 1), the others every brick of
+        the layout, ghost bricks included (skip = 0).  Returns the array
+        holding the result."""
+        every = np.arange(self.nbricks, dtype=np.uint32)
+        src, dst = a, b
+        for t in range(n):
+            if t % ghost_iters == 0:
+                self.refresh(src, dtype)
+            last = t % ghost_iters == ghost_iters - 1 or t == n - 1
+            self.sweep(stencil, coeff, src, dst, dtype, mode, theta, cond_mode,
+                       blist=None if last else every)
+            src, dst = dst, src
+        return src
 
     def storage(self, dtype=F64, fields=1):
         return np.zeros((self.nbricks, 512 * fields), NP[dtype])
-
-    def fill_uniform(self, dat, seed, lo=-1.0, hi=1.0, dtype=F64, step=None, offset=0):
-        step = step or dat.shape[1]
-        base = dat.reshape(-1)[offset:]
-        lib().or_fill_uniform(dtype, P(base), step, L3(self.nb), seed, lo, hi)
 
     def refresh(self, dat, dtype=F64, offset=0):
         base = dat.reshape(-1)[offset:]
@@ -166,6 +179,22 @@ class Grid:
             self.sweep(stencil, coeff, src, dst, dtype, mode, theta, cond_mode)
             src, dst = dst, src
         return src
+
+
+class Grid(Layout):
+    """Host twin of bricklib_amd.PeriodicGrid: (nb+2)^3 bricks, one ghost
+    layer, numbering/adjacency of init_grid."""
+
+    def __init__(self, nb):
+        self.nb = tuple(int(x) for x in nb)
+        self.padded = tuple(x + 2 for x in self.nb)
+        self.grid, adj = init_grid(self.padded)
+        super().__init__(adj, interior_list(self.nb), ghost_pairs(self.nb))
+
+    def fill_uniform(self, dat, seed, lo=-1.0, hi=1.0, dtype=F64, step=None, offset=0):
+        step = step or dat.shape[1]
+        base = dat.reshape(-1)[offset:]
+        lib().or_fill_uniform(dtype, P(base), step, L3(self.nb), seed, lo, hi)
 
     def to_dense(self, dat, offset=0):
         """interior field as a dense (n2, n1, n0) array (x fastest)."""

@@ -151,3 +151,31 @@ def test_rejects_bad_arguments():
         DecompLayout((1, 4, 4))  # fewer than 2 * ghost bricks on an axis
     with pytest.raises(BrickError):
         DecompLayout((4, 4, 4), brick_bytes=3000)  # not a factor of the page
+
+
+@pytest.mark.parametrize("skin", ["good", "normal", "bad"])
+@pytest.mark.parametrize("name", ["7pt", "25pt", "125pt", "cond"])
+def test_oracle_on_decomp_numbering_matches_init_grid(name, skin):
+    """The oracle's brick loop on the BrickDecomp numbering (its adjacency
+    and its exchange pairs as the refresh) gives the init_grid numbering's
+    values brick for brick: the layout is a valid periodic grid."""
+    import oracle as O
+    from bricklib_amd import stencils as S
+    from bricklib_amd.decomp import SKIN3D_BAD, SKIN3D_NORMAL, DecompLayout
+    nb = (4, 3, 5)
+    L = DecompLayout(nb, skin={"good": None, "normal": SKIN3D_NORMAL, "bad": SKIN3D_BAD}[skin])
+    st = S.Stencil.make(name, seed=43)
+    og = O.Grid(nb)
+    oa = og.storage()
+    og.fill_uniform(oa, 19)
+    lay = O.Layout(L.adj, L.interior(), L.periodic_pairs())
+    src = lay.storage()
+    src[L.grid.reshape(-1)] = oa[og.grid]
+    got = lay.sweeps(st.kind, st.coeff, src, lay.storage(), 5, theta=st.theta,
+                     cond_mode=st.cond_mode)
+    want = og.sweeps(st.kind, st.coeff, oa, og.storage(), 5, theta=st.theta,
+                     cond_mode=st.cond_mode)
+    pk, pj, pi = og.padded[2], og.padded[1], og.padded[0]
+    inner = og.grid.reshape(pk, pj, pi)[1:-1, 1:-1, 1:-1].reshape(-1)
+    dinner = L.grid[1:-1, 1:-1, 1:-1].reshape(-1)
+    np.testing.assert_array_equal(got[dinner], want[inner])

@@ -174,3 +174,45 @@ def test_malformed_requests_rejected(gpu):
         Batch(0)
     with pytest.raises(ValueError):
         serve.Server(hbm_budget=1000).serve([r], realtime=False)
+
+
+def test_serving_stream_at_workload_parameters(gpu, orc):
+    """The serving workload's own parameters: Poisson arrivals at 200 req/s,
+    each grid edge uniform over the multiples of 8 in [32, 256], stencil
+    uniform over {7, 13, 25, 125}-point, FP64 or FP32 with probability 1/2,
+    1..200 sweeps; two lanes, pipelined chunks.  A sample of the requests
+    (the first ones, the largest, both dtypes, every stencil) gives its
+    single-grid result bit for bit, and the small FP64 ones the oracle's."""
+    reqs = serve.workload(64, rate=200.0, seed=11)
+    assert {r.stencil.kind for r in reqs} == {0, 1, 2, 3}
+    assert {r.dtype for r in reqs} == {BK_F64, BK_F32}
+    assert min(min(r.nb) for r in reqs) >= 4 and max(max(r.nb) for r in reqs) <= 32
+    by_work = sorted(reqs, key=lambda r: r.cells * r.steps)
+    sample = {r.rid for r in reqs[:4]} | {r.rid for r in by_work[-3:]}
+    for kind in range(4):
+        for dt in (BK_F64, BK_F32):
+            hit = [r for r in reqs if r.stencil.kind == kind and r.dtype == dt]
+            if hit:
+                sample.add(hit[0].rid)
+    small64 = [r for r in by_work if r.dtype == BK_F64][:3]
+    sample |= {r.rid for r in small64}
+    srv = serve.Server()
+    stats = srv.serve(reqs, realtime=True, keep_results=sample)
+    s = stats.summary()
+    print(s)
+    assert s["requests"] == len(reqs) and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+    for r in reqs:
+        if r.rid not in sample:
+            continue
+        print(f"[case] rid={r.rid} {r.nb} kind={r.stencil.kind} dtype={r.dtype} "
+              f"steps={r.steps}", flush=True)
+        np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
+    for r in small64:
+        og = O.Grid(r.nb)
+        oa, ob = og.storage(), og.storage()
+        og.fill_uniform(oa, r.seed)
+        ores = og.sweeps(r.stencil.kind, r.stencil.coeff, oa, ob, r.steps,
+                         theta=r.stencil.theta, cond_mode=r.stencil.cond_mode)
+        np.testing.assert_array_equal(r.interior_host(), ores[og.interior])
+    for r in reqs:
+        r.release()

@@ -37,3 +37,23 @@ def test_decomp_example(gpu, n, sweeps, skin):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "bit-identical" in r.stdout and "result match" in r.stdout
+
+
+HOOK_EXE = os.path.join(ROOT, "examples", "bin", "brick_hook_7pt")
+
+
+@pytest.mark.parametrize("n,sweeps", [(32, 1), (64, 4), (40, 3)])
+def test_brick_hook_example(gpu, n, sweeps):
+    """examples/brick_hook_7pt.cpp keeps the reference's per-brick
+    brick("7pt.py", ...) call site: on host bricks it computes at once, on
+    device-resident bricks it queues them and runs one GPU sweep at the next
+    flush; both against the array loop (compareBrick) and against each other
+    bit for bit, over ping-pong sweeps too; tile(...) against the array
+    loop."""
+    assert os.path.exists(HOOK_EXE), "examples/bin/brick_hook_7pt not built (run build())"
+    print(f"[case] brick_hook_7pt n={n} sweeps={sweeps}", flush=True)
+    r = subprocess.run([HOOK_EXE, str(n), str(sweeps)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "result match" in r.stdout

@@ -1,0 +1,293 @@
+"""Parity at the workloads' own sizes, through the C-ABI on the GPU.
+
+(a) 100 ping-pong FP64 sweeps with the periodic refresh at 128^3 for every
+    stencil: bit-exact against the oracle's fixed (FMA) order, and within
+    1e-10 per point of the oracle's reference-order evaluation
+    (stencils/3axis.cpp:60-63 and the .py tap files), one sweep within
+    1e-12.  Relative error per point is |g - r| / max(|r|, rms(r)): the
+    floor is the RMS of the reference field itself, so the bound scales
+    with a field that decays over the sweeps (max |field| is ~0.03 for the
+    7-point star and ~1e-60 for the cube after 100 sweeps) instead of being
+    an absolute bound that a decayed field could never violate; points
+    where the field crosses zero are held to the field's typical size.
+(b) Full 512^3 FP64 grids (the 66^3-brick padded grid of 287 496 bricks):
+    one sweep of every stencil bit-exact against the oracle; 100 sweeps
+    against closed forms -- a periodic plane wave exp(i k.x) is an
+    eigenvector of every linear periodic stencil with eigenvalue
+    lam = sum_t c_t exp(i k.o_t), so S sweeps of cos(k.x) give
+    |lam|^S cos(k.x + S arg lam) exactly; a constant field v gives
+    v (sum c)^S, and for cond the select rule applied S times.
+(c) FP32 7-point and 125-point at 1024^3 (2 197 000 bricks, 2^30 interior
+    points; the cube's quad schedule addresses 2^30.2 elements through its
+    32-bit offsets) against the same closed forms, error bound 1e-5.
+
+Dense fields for the closed forms are built and compared with torch on the
+device (plumbing only); every sweep and every array<->brick conversion is a
+call into libbricks.so.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bricklib_amd import BK_F32, BK_F64, stencils as S
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch ships with the image
+    torch = None
+
+pytestmark = pytest.mark.gpu
+
+NAMES = {0: "7pt", 1: "13pt", 2: "25pt", 3: "125pt", 4: "cond"}
+LINEAR = [S.BK_STAR7, S.BK_STAR13, S.BK_STAR25, S.BK_CUBE125]
+CASES = [(S.BK_STAR7, 1), (S.BK_STAR13, 1), (S.BK_STAR25, 1), (S.BK_CUBE125, 1),
+         (S.BK_COND7, 0), (S.BK_COND7, 1)]  # (stencil, cond mode)
+
+
+def case_id(c):
+    return NAMES[c[0]] + ("" if c[0] != S.BK_COND7 else ("-ref" if c[1] == 0 else "-theta"))
+
+
+def make_stencil(kind, cond_mode=1, seed=3):
+    return S.Stencil.make(kind, seed=seed, cond_mode=cond_mode)
+
+
+def rel_rms_floor(got, ref):
+    """max |g - r| / max(|r|, rms(r)) over the arrays (numpy, FP64)."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    rms = float(np.sqrt(np.mean(ref * ref)))
+    den = np.maximum(np.abs(ref), rms)
+    return float(np.max(np.abs(got - ref) / den)), rms
+
+
+def gpu_run(gpu, nb, st, dtype, seed, nsweeps):
+    g = gpu.PeriodicGrid(nb)
+    a = gpu.Brick(g.info, g.field_storage(dtype), 0)
+    b = gpu.Brick(g.info, g.field_storage(dtype), 0)
+    g.fill_uniform(a, seed)
+    b.storage.buf.zero()
+    res = gpu.sweeps(st, a, b, g, nsweeps)
+    return g, res.storage.to_host()
+
+
+# ---------------------------------------------------------------------------
+# (a) 100 sweeps at 128^3 against both oracle orders
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", CASES, ids=case_id)
+def test_100_sweeps_128_fp64(gpu, orc, case):
+    kind, cmode = case
+    print(f"[case] 100 sweeps FP64 128^3 {case_id(case)}", flush=True)
+    st = make_stencil(kind, cmode)
+    nb = (16, 16, 16)
+    g, got = gpu_run(gpu, nb, st, BK_F64, 41, 100)
+    og = O.Grid(nb)
+    for mode in (O.MODE_FMA, O.MODE_REF):
+        oa, ob = og.storage(), og.storage()
+        og.fill_uniform(oa, 41)
+        ores = og.sweeps(kind, st.coeff, oa, ob, 100, mode=mode, theta=st.theta,
+                         cond_mode=st.cond_mode)
+        if mode == O.MODE_FMA:
+            np.testing.assert_array_equal(got[og.interior], ores[og.interior])
+        else:
+            err, rms = rel_rms_floor(got[og.interior], ores[og.interior])
+            print(f"[err] {case_id(case)} 100 sweeps vs reference order: max rel {err:.3e} "
+                  f"(floor rms {rms:.3e})")
+            assert err <= 1e-10
+
+
+@pytest.mark.parametrize("case", CASES, ids=case_id)
+def test_one_sweep_128_fp64_reference_order(gpu, orc, case):
+    kind, cmode = case
+    st = make_stencil(kind, cmode, seed=8)
+    nb = (16, 16, 16)
+    g, got = gpu_run(gpu, nb, st, BK_F64, 42, 1)
+    og = O.Grid(nb)
+    oa, ob = og.storage(), og.storage()
+    og.fill_uniform(oa, 42)
+    ores = og.sweeps(kind, st.coeff, oa, ob, 1, mode=O.MODE_REF, theta=st.theta,
+                     cond_mode=st.cond_mode)
+    err, rms = rel_rms_floor(got[og.interior], ores[og.interior])
+    print(f"[err] {case_id(case)} 1 sweep vs reference order: max rel {err:.3e} (rms {rms:.3e})")
+    assert err <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# (b) full 512^3: one sweep bit-exact against the oracle
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", CASES, ids=case_id)
+def test_one_sweep_512_fp64_bit_exact(gpu, orc, case):
+    kind, cmode = case
+    print(f"[case] one sweep FP64 512^3 {case_id(case)}", flush=True)
+    st = make_stencil(kind, cmode, seed=5)
+    nb = (64, 64, 64)
+    g, got = gpu_run(gpu, nb, st, BK_F64, 77, 1)
+    og = O.Grid(nb)
+    oa = og.storage()
+    og.fill_uniform(oa, 77)
+    ob = og.storage()
+    ores = og.sweeps(kind, st.coeff, oa, ob, 1, theta=st.theta, cond_mode=st.cond_mode)
+    del oa
+    assert np.array_equal(got[og.interior], ores[og.interior])
+    # and within 1e-12 of the reference order
+    oa = og.storage()
+    og.fill_uniform(oa, 77)
+    ob.fill(0)
+    rref = og.sweeps(kind, st.coeff, oa, ob, 1, mode=O.MODE_REF, theta=st.theta,
+                     cond_mode=st.cond_mode)
+    e = O.field_compare(got, rref, og.interior, floor=0.0)
+    rms = math.sqrt(e["sum_sq_ref"] / (og.interior.size * 512))
+    e = O.field_compare(got, rref, og.interior, floor=rms)
+    print(f"[err] {case_id(case)} 512^3 one sweep vs reference order: {e['max_rel']:.3e}")
+    assert e["max_rel"] <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# closed forms on the device
+# ---------------------------------------------------------------------------
+class _Ptr:
+    def __init__(self, p):
+        self.ptr = int(p)
+
+
+def _coords(n, dev):
+    return torch.arange(-8, n + 8, device=dev, dtype=torch.float64).remainder(n)
+
+
+def dense_wave(nb, k, dtype, value=None):
+    """The padded dense array (n2+16, n1+16, n0+16), x fastest, of cos(k.x)
+    (or the constant `value`) over periodic coordinates, on cuda:0."""
+    n0, n1, n2 = (8 * x for x in nb)
+    dev = torch.device("cuda", 0)
+    tdt = torch.float64 if dtype == BK_F64 else torch.float32
+    if value is not None:
+        return torch.full((n2 + 16, n1 + 16, n0 + 16), float(value), dtype=tdt, device=dev)
+    ax = (2 * math.pi * k[0] / n0) * _coords(n0, dev)
+    ay = (2 * math.pi * k[1] / n1) * _coords(n1, dev)
+    az = (2 * math.pi * k[2] / n2) * _coords(n2, dev)
+    phase = az[:, None, None] + ay[None, :, None] + ax[None, None, :]
+    return torch.cos(phase).to(tdt).contiguous()
+
+
+def eigen(st, nb, k, dtype=BK_F64):
+    """lam = sum_t c_t exp(i k.o_t) for the stencil's taps, with the
+    coefficients the kernel computes with (rounded to FP32 for FP32)."""
+    n = [8 * x for x in nb]
+    coeff = np.asarray(st.coeff, np.float64)
+    if dtype != BK_F64:
+        coeff = coeff.astype(np.float32).astype(np.float64)
+    lam = 0j
+    for c, (di, dj, dk) in zip(coeff, S.offsets(st.kind)):
+        ph = 2 * math.pi * (k[0] * di / n[0] + k[1] * dj / n[1] + k[2] * dk / n[2])
+        lam += c * complex(math.cos(ph), math.sin(ph))
+    return lam
+
+
+def expected_wave(nb, k, lam, nsweeps):
+    """|lam|^S cos(k.x + S arg lam) over the interior, FP64 on cuda:0."""
+    n0, n1, n2 = (8 * x for x in nb)
+    dev = torch.device("cuda", 0)
+    ax = (2 * math.pi * k[0] / n0) * torch.arange(n0, device=dev, dtype=torch.float64)
+    ay = (2 * math.pi * k[1] / n1) * torch.arange(n1, device=dev, dtype=torch.float64)
+    az = (2 * math.pi * k[2] / n2) * torch.arange(n2, device=dev, dtype=torch.float64)
+    ang = (nsweeps * math.atan2(lam.imag, lam.real)) % (2 * math.pi)
+    phase = az[:, None, None] + ay[None, :, None] + ax[None, None, :] + ang
+    return abs(lam) ** nsweeps * torch.cos(phase)
+
+
+def sweep_dense(gpu, nb, st, dtype, dense, nsweeps):
+    """Bricks from the dense array (bk_copy_to_brick), nsweeps ping-pong
+    sweeps with refresh, back to a dense interior array
+    (bk_copy_from_brick); returns the interior as FP64."""
+    g = gpu.PeriodicGrid(nb)
+    a = gpu.Brick(g.info, g.field_storage(dtype), 0)
+    b = gpu.Brick(g.info, g.field_storage(dtype), 0)
+    dims = [8 * x for x in nb]
+    torch.cuda.synchronize()
+    gpu.copyToBrick(dims, (0, 0, 0), (8, 8, 8), _Ptr(dense.data_ptr()), g.d_grid, a)
+    gpu.device_sync()
+    res = gpu.sweeps(st, a, b, g, nsweeps)
+    gpu.device_sync()
+    out = torch.empty_like(dense)
+    gpu.copyFromBrick(dims, (0, 0, 0), (8, 8, 8), _Ptr(out.data_ptr()), g.d_grid, res)
+    gpu.device_sync()
+    del a, b, g
+    return out[8:-8, 8:-8, 8:-8].to(torch.float64)
+
+
+def wave_error(got, exp):
+    rms = float(torch.sqrt(torch.mean(exp * exp)))
+    den = torch.clamp(exp.abs(), min=rms)
+    return float(((got - exp).abs() / den).max()), rms
+
+
+WAVES = [(0, 0, 0), (1, 2, 3), (5, 0, 7)]
+
+
+@pytest.mark.skipif(torch is None, reason="torch is the device-array plumbing of this test")
+@pytest.mark.parametrize("kind", LINEAR, ids=lambda s: NAMES[s])
+def test_plane_waves_512_fp64_100_sweeps(gpu, kind):
+    nb = (64, 64, 64)
+    st = make_stencil(kind, seed=6)
+    for k in WAVES:
+        print(f"[case] plane wave k={k} {NAMES[kind]} FP64 512^3 100 sweeps", flush=True)
+        dense = dense_wave(nb, k, BK_F64)
+        got = sweep_dense(gpu, nb, st, BK_F64, dense, 100)
+        lam = eigen(st, nb, k)
+        exp = expected_wave(nb, k, lam, 100)
+        err, rms = wave_error(got, exp)
+        print(f"[err] {NAMES[kind]} k={k} |lam|={abs(lam):.6f} max rel {err:.3e} "
+              f"(rms {rms:.3e})")
+        assert err <= 1e-10
+        del dense, got, exp
+
+
+@pytest.mark.skipif(torch is None, reason="torch is the device-array plumbing of this test")
+@pytest.mark.parametrize("cmode", [0, 1], ids=["ref", "theta"])
+@pytest.mark.parametrize("v", [0.625, -0.375])
+def test_cond_constant_field_512_fp64_100_sweeps(gpu, cmode, v):
+    """cond on a constant field v: every point sees p = (sum c) max(v, 0)
+    and selects p or -p by its rule (p > 0, or v > theta); applied 100
+    times."""
+    nb = (64, 64, 64)
+    st = make_stencil(S.BK_COND7, cmode, seed=6)
+    dense = dense_wave(nb, None, BK_F64, value=v)
+    got = sweep_dense(gpu, nb, st, BK_F64, dense, 100)
+    csum = float(np.sum(st.coeff))
+    x = v
+    for _ in range(100):
+        p = csum * max(x, 0.0)
+        x = p if ((p > 0) if cmode == 0 else (x > st.theta)) else -p
+    print(f"[case] cond constant {v} mode {cmode}: expect {x!r}", flush=True)
+    assert float((got - x).abs().max()) <= 1e-12 * max(abs(x), 1e-300) + 0.0
+
+
+@pytest.mark.skipif(torch is None, reason="torch is the device-array plumbing of this test")
+@pytest.mark.parametrize("kind,nsweeps", [(S.BK_STAR7, 100), (S.BK_CUBE125, 10)],
+                         ids=["7pt-100", "125pt-10"])
+def test_plane_waves_1024_fp32(gpu, kind, nsweeps):
+    """FP32 bricks at 1024^3 (4.5 GB per field).  The cube takes coefficients
+    U(0,1) normalised to sum 1 here so a low wave keeps |lam| near 1 and the
+    field stays far above FP32's range limits over the sweeps; the error
+    bound is 1e-5 of the field's RMS."""
+    nb = (128, 128, 128)
+    st = make_stencil(kind, seed=9)
+    if kind == S.BK_CUBE125:
+        c = np.array([S.u01(9, q) for q in range(125)])
+        st = S.Stencil(kind, c / c.sum())
+    for k in [(0, 0, 0), (1, 2, 3)]:
+        print(f"[case] plane wave k={k} {NAMES[kind]} FP32 1024^3 {nsweeps} sweeps", flush=True)
+        dense = dense_wave(nb, k, BK_F32)
+        got = sweep_dense(gpu, nb, st, BK_F32, dense, nsweeps)
+        del dense
+        lam = eigen(st, nb, k, BK_F32)
+        exp = expected_wave(nb, k, lam, nsweeps)
+        err, rms = wave_error(got, exp)
+        print(f"[err] FP32 {NAMES[kind]} k={k} |lam|={abs(lam):.6f} max rel {err:.3e} "
+              f"(rms {rms:.3e})")
+        assert err <= 1e-5
+        del got, exp
+        torch.cuda.empty_cache()

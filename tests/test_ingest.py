@@ -11,43 +11,9 @@ import oracle as O
 from bricklib_amd import ingest
 from bricklib_amd._lib import BK_COND7, BK_CUBE125, BK_STAR7, BK_STAR13, BK_STAR25
 
+from _dsl import HEAD, star, text_of
+
 REF = "/root/reference/stencils"
-
-HEAD = """from st.expr import Index, ConstRef, If
-from st.grid import Grid
-from st.func import Func
-i = Index(0)
-j = Index(1)
-k = Index(2)
-src = Grid("u", 3)
-dst = Grid("v", 3)
-"""
-
-
-def acc(d):
-    di, dj, dk = d
-    f = lambda n, v: n if v == 0 else (f"{n} + {v}" if v > 0 else f"{n} - {-v}")  # noqa: E731
-    return f"src({f('i', di)}, {f('j', dj)}, {f('k', dk)})"
-
-
-def text_of(taps, relu=False, select=False):
-    """DSL text for [(offset, const_name_or_literal)] in the given order."""
-    lines = [HEAD, "maxf = Func(\"max\", 2)", "zero = ConstRef(\"0.0\")"]
-    names = {}
-    for _, c in taps:
-        if c not in names:
-            names[c] = f"q{len(names)}"
-            lines.append(f"{names[c]} = ConstRef(\"{c}\")")
-    terms = []
-    for d, c in taps:
-        x = f"maxf({acc(d)}, zero)" if relu else acc(d)
-        terms.append(f"{names[c]} * {x}")
-    lines.append("calc = " + " + \\\n    ".join(terms))
-    if select:
-        lines.append("calc = If(calc > 0, calc, -calc)")
-    lines.append("dst(i, j, k).assign(calc)")
-    lines.append("STENCIL = [dst]")
-    return "\n".join(lines) + "\n"
 
 
 def brute(field, spec, values, relu=False, select=False):
@@ -72,13 +38,6 @@ def oracle_sweep(spec, values, nb=(3, 2, 4)):
     g.refresh(a)
     g.sweep(st.kind, st.coeff, a, b, mode=O.MODE_FMA, cond_mode=0)
     return inp, g.to_dense(b)
-
-
-def star(R):
-    offs = [(0, 0, 0)]
-    for r in range(1, R + 1):
-        offs += [(r, 0, 0), (-r, 0, 0), (0, r, 0), (0, -r, 0), (0, 0, r), (0, 0, -r)]
-    return offs
 
 
 @pytest.mark.parametrize("R,kind", [(1, BK_STAR7), (2, BK_STAR13), (4, BK_STAR25)])

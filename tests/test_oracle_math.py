@@ -220,3 +220,32 @@ def test_bf16_rounding():
     assert O.f32_to_bf16(np.array([1.00390625], np.float32))[0] == 0x3F80
     back = O.bf16_to_f32(got)
     assert np.all(np.abs(back[:4] - vals[:4]) <= np.abs(vals[:4]) * 2 ** -8)
+
+
+@pytest.mark.parametrize("name,gi", [("7pt", 8), ("13pt", 4), ("25pt", 2), ("125pt", 4),
+                                     ("cond", 8)])
+def test_oracle_deep_ghost_loop_interior_exact(name, gi):
+    """The communication-avoiding loop (weak/main.cpp:26-36,203-211) keeps the
+    interior exact while radius * ghost_iters <= 8: stale ghost values move
+    inwards one radius per sweep and never reach the interior before the
+    next refresh.  One sweep more per group does reach it (checked for the
+    linear stencils; cond's select can hide a stale value)."""
+    from bricklib_amd import stencils as S
+    nb = (3, 2, 4)
+    st = S.Stencil.make(name, seed=23)
+    g = O.Grid(nb)
+    n = 2 * gi + 3
+    outs = []
+    for k in (1, gi):
+        a, b = g.storage(), g.storage()
+        g.fill_uniform(a, 5)
+        outs.append(g.deep_sweeps(st.kind, st.coeff, a, b, n, k, theta=st.theta,
+                                  cond_mode=st.cond_mode)[g.interior])
+    np.testing.assert_array_equal(outs[1], outs[0])
+    if name == "cond":
+        return
+    a, b = g.storage(), g.storage()
+    g.fill_uniform(a, 5)
+    over = g.deep_sweeps(st.kind, st.coeff, a, b, n, gi + 1, theta=st.theta,
+                         cond_mode=st.cond_mode)[g.interior]
+    assert not np.array_equal(over, outs[0])
