@@ -16,8 +16,10 @@ The same JSON line also carries:
                   (16 B/point: one FP64 read + one write) / its mean launch
                   time, against the 8 TB/s HBM3E peak; `measured_copy_gbs`
                   is a plain device copy kernel on the same box;
-  cpu_baseline -- the oracle's restatement of the reference brick loop
-                  (stencils/3axis.cpp:51-66) with OpenMP on the host cores;
+  cpu_baseline -- the oracle's restatement of the reference's CPU brick
+                  loops (stencils/3axis.cpp:51-66 and the reference-order
+                  13/25/125-point and cond loops) with OpenMP on the host
+                  cores, per stencil, with the CPU model and core counts;
   by_stencil   -- 13/25-point, 125-point cube and cond at the same size,
                   plus the FP32 / BF16 forms of the 7-point star;
   precision    -- FP32 and BF16-storage forms against the FP64 sweep after
@@ -62,7 +64,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--n", type=int, default=512, help="interior cells per axis (multiple of 8)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-seconds", type=float, default=20.0,
+                   help="host seconds for the CPU baseline, split over its five stencils")
     p.add_argument("--only", default="", help="comma list of extra cases to run (default all)")
     p.add_argument("--no-extra", action="store_true")
     p.add_argument("--no-precision", action="store_true")
@@ -181,38 +184,80 @@ def valu_peak(bk, stream, warm_s=0.3):
     return best
 
 
-def cpu_baseline(nb, coeff, seconds):
-    """Oracle brick loop (reference order, 3axis.cpp:51-66) on the host."""
+def host_cpu():
+    """CPU model, physical cores and logical CPUs of this host (/proc/cpuinfo)."""
+    model, phys, logical = "unknown", set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    cid = v
+                    phys.add((pid, cid))
+    except OSError:
+        pass
+    return {"cpu_model": model, "physical_cores": len(phys) or None, "logical_cpus": logical,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(nb, seconds):
+    """The oracle's restatement of the reference's CPU brick loops on the
+    host, OpenMP over the interior bricks (stencils/3axis.cpp:51-66 for the
+    7-point star through the Brick accessor; the reference-order loops of
+    the 13/25/125-point and cond stencils), each timed like time_func
+    (stencils/stencils.h:40-53: one warm-up, then whole sweeps with the
+    periodic refresh until its share of `seconds` has passed), over the full
+    grid.  `value` is the 7-point star's rate."""
     import oracle as O
-    import ctypes as C
+    from bricklib_amd import stencils as S
     g = O.Grid(nb)
     a = g.storage()
     b = g.storage()
     g.fill_uniform(a, 1234)
     g.refresh(a)
-    c = np.ascontiguousarray(coeff)
-    fn = O.lib().or_brick_sweep_7pt_accessor
-
-    def sweep(src, dst):
-        fn(O.P(g.adj, O.u32p), O.P(g.interior, O.u32p), g.interior.size, O.P(src, O.dblp), 512,
-           O.P(dst, O.dblp), 512, O.P(c, O.dblp))
-    sweep(a, b)  # warm-up
-    n = 0
-    t0 = time.perf_counter()
-    src, dst = a, b
-    while True:
-        g.refresh(src)
-        sweep(src, dst)
-        src, dst = dst, src
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 1000:
-            break
+    acc = O.lib().or_brick_sweep_7pt_accessor
     pts = g.interior.size * 512
-    return {"value": pts * n / el / 1e9, "unit": "GStencil/s",
-            "cores": int(O.lib().or_num_threads()), "kind": "port",
-            "sample": f"{n} sweeps (periodic refresh + 7-pt brick loop) over the full "
-                      f"{8 * nb[0]}^3 FP64 grid, {el:.1f} s"}
+    by = {}
+    names = ("7pt", "13pt", "25pt", "125pt", "cond")
+    for name in names:
+        st = S.Stencil.make(name, seed=1)
+        c = np.ascontiguousarray(st.coeff)
+
+        def sweep(src, dst):
+            if name == "7pt":
+                acc(O.P(g.adj, O.u32p), O.P(g.interior, O.u32p), g.interior.size,
+                    O.P(src, O.dblp), 512, O.P(dst, O.dblp), 512, O.P(c, O.dblp))
+            else:
+                g.sweep(st.kind, c, src, dst, mode=O.MODE_REF, theta=st.theta,
+                        cond_mode=st.cond_mode)
+        sweep(a, b)  # warm-up
+        n, src, dst = 0, a, b
+        t0 = time.perf_counter()
+        while True:
+            g.refresh(src)
+            sweep(src, dst)
+            src, dst = dst, src
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds / len(names) or n >= 1000:
+                break
+        by[name] = {"gstencil_s": round(pts * n / el / 1e9, 4), "sweeps": n,
+                    "seconds": round(el, 2)}
+    info = host_cpu()
+    threads = int(O.lib().or_num_threads())
+    return {"value": by["7pt"]["gstencil_s"], "unit": "GStencil/s", "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 {8 * nb[0]}^3 grid, whole sweeps with the periodic refresh, "
+                      f"{seconds / len(names):.1f} s per stencil (value: the 7-point star)",
+            "threads": threads, **info, "by_stencil": by}
 
 
 def serve_bench(bk, S, args):
@@ -224,8 +269,11 @@ def serve_bench(bk, S, args):
     from bricklib_amd import BK_F32, BK_F64, precision as PR, serve as SV
     from bricklib_amd.batch import Request
     out = {}
-    # single-grid reference rates at 512^3 (refresh + sweep, default schedule)
-    g = bk.PeriodicGrid((64, 64, 64))
+    # single-grid reference rates at 512^3 in the serving loop's own field
+    # layout (the wrapped adjacency: no ghost bricks, no refresh), default
+    # schedule, so the efficiency compares like with like
+    serve_ghost = 0
+    g = bk.PeriodicGrid((64, 64, 64), ghost=serve_ghost)
     stream = bk.Stream()
     rates = {}
     for name in ("7pt", "13pt", "25pt", "125pt"):
@@ -237,19 +285,23 @@ def serve_bench(bk, S, args):
             # warmed like the by-stencil cases: a cold GPU would understate
             # the single-grid rates (the FP64 cube by about a third) and so
             # overstate the batched efficiency
-            ms = time_sweeps(bk, st, fa, fb, g, stream, 10, 2, warm_s=args.warm_s)
+            ms = time_sweeps(bk, st, fa, fb, g, stream, 10, 2, refresh=bool(serve_ghost),
+                             warm_s=args.warm_s)
             rates[(S.NAMES[name], dt)] = g.cells / (ms * 1e-3)
             del fa, fb
     del g
     out["single_grid_gstencil_s"] = {f"{SV.stencil_name(k)}_{PR.DTYPE_NAMES[d]}": round(v / 1e9, 2)
                                      for (k, d), v in rates.items()}
+    out["layout"] = ("wrapped adjacency, no ghost bricks or refresh, for the batched requests "
+                     "and the single-grid rates alike" if serve_ghost == 0 else
+                     "one-brick ghost zone refreshed every sweep, batched and single grid alike")
 
     # real-time Poisson stream
     reqs = SV.workload(args.serve_requests, rate=args.serve_rate, seed=2024)
     # chunks of 16 ticks: at this load a chunk's GPU time is short and the
     # host's round trip per chunk adds to every request's latency (p50 10.5
     # ms with chunks of 8, 8.5-9.0 with 16 or 32, 11 with 64)
-    srv = SV.Server(hbm_budget=200e9, chunk=16)
+    srv = SV.Server(hbm_budget=200e9, chunk=16, ghost=serve_ghost)
     srv.prewarm(reqs)
     sample = set(r.rid for r in reqs[::max(1, len(reqs) // 12)])
     st = srv.serve(reqs, realtime=True, keep_results=sample)
@@ -340,7 +392,8 @@ def main():
     # periodic ghost bricks of its output (the input's ghosts are refreshed
     # once first): bk_sweeps_fused, the same bits as a separate refresh
     # before every sweep (that form is by_stencil's 7pt_f64_separate_refresh)
-    ms_step = time_sweeps(bk, st7, a, b, g, stream, args.steps, args.warmup, fused=True)
+    ms_step = time_sweeps(bk, st7, a, b, g, stream, args.steps, args.warmup, fused=True,
+                          warm_s=args.warm_s)
     # the dominant kernel alone, for the roofline
     ms_kernel = time_sweeps(bk, st7, a, b, g, stream, args.steps, 2, refresh=False)
     peak_copy, copies = copy_peak(bk, stream)
@@ -466,7 +519,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(nb, st7.coeff, args.cpu_seconds)
+        cpu = cpu_baseline(nb, args.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     value = pts / (ms_step * 1e-3) / 1e9
