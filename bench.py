@@ -70,10 +70,14 @@ def parse():
     p.add_argument("--no-extra", action="store_true")
     p.add_argument("--no-precision", action="store_true")
     p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--no-tune", action="store_true",
+                   help="skip the autotuner table (brick shapes, folds, schedules at n^3)")
     p.add_argument("--warm-s", type=float, default=WARM_S,
                    help="seconds of untimed sweeps before each by-stencil and precision timing")
     p.add_argument("--serve-requests", type=int, default=600)
     p.add_argument("--serve-rate", type=float, default=200.0)
+    p.add_argument("--serve-tune", type=int, default=0,
+                   help="1: each request's schedule from the autotuner per size class")
     p.add_argument("--precision-1024", action="store_true", default=True,
                    help="also run the precision study at twice the size (1024^3)")
     return p.parse_args()
@@ -301,14 +305,18 @@ def serve_bench(bk, S, args):
     # chunks of 16 ticks: at this load a chunk's GPU time is short and the
     # host's round trip per chunk adds to every request's latency (p50 10.5
     # ms with chunks of 8, 8.5-9.0 with 16 or 32, 11 with 64)
-    srv = SV.Server(hbm_budget=200e9, chunk=16, ghost=serve_ghost)
+    srv = SV.Server(hbm_budget=200e9, chunk=16, ghost=serve_ghost, tune=args.serve_tune)
+    t_pw = time.perf_counter()
     srv.prewarm(reqs)
+    out["prewarm_s"] = round(time.perf_counter() - t_pw, 3)
     sample = set(r.rid for r in reqs[::max(1, len(reqs) // 12)])
     st = srv.serve(reqs, realtime=True, keep_results=sample)
     rt = st.summary()
     rt["rate_req_s"] = args.serve_rate
     rt["offered_gstencil_s"] = round(sum(r.cells * r.steps for r in reqs) /
                                      max(reqs[-1].arrival, 1e-9) / 1e9, 2)
+    rt["schedules"] = sorted({f"{SV.stencil_name(r.stencil.kind)}:{r.schedule[0]}/{r.schedule[1]}"
+                              for r in reqs})
     out["poisson"] = rt
     print(f"[bench] serving poisson: {rt}", file=sys.stderr, flush=True)
 
@@ -320,7 +328,7 @@ def serve_bench(bk, S, args):
         ref = Request(rid=-1, nb=r.nb, stencil=r.stencil, dtype=BK_F64, steps=r.steps,
                       seed=r.seed)
         srv.serve([ref], realtime=False, keep_results=True)
-        gr = r.shape.grid
+        gr = bk.PeriodicGrid(r.nb, ghost=serve_ghost)
         a = bk.Brick(gr.info, bk.BrickStorage(_View(r.result_ptr, r.field_bytes), gr.nbricks,
                                               512, r.dtype), 0)
         b = bk.Brick(gr.info, bk.BrickStorage(_View(ref.result_ptr, ref.field_bytes),
@@ -344,6 +352,24 @@ def serve_bench(bk, S, args):
     sat["efficiency_vs_single_grid"] = round(single_time / st2.elapsed, 4)
     out["saturating"] = sat
     print(f"[bench] serving saturating: {sat}", file=sys.stderr, flush=True)
+
+    # the same saturating mix with the other schedule policy (autotuned per
+    # size class, or the fixed defaults)
+    alt = not args.serve_tune
+    srv2 = SV.Server(hbm_budget=200e9, chunk=16, ghost=serve_ghost, tune=alt)
+    t_pw = time.perf_counter()
+    reqs3 = SV.workload(args.serve_requests // 2, rate=args.serve_rate, seed=77)
+    srv2.prewarm(reqs3)
+    pw = time.perf_counter() - t_pw
+    st3 = srv2.serve(reqs3, realtime=False)
+    sat2 = st3.summary()
+    sat2["prewarm_s"] = round(pw, 3)
+    sat2["efficiency_vs_single_grid"] = round(single_time / st3.elapsed, 4)
+    sat2["schedules"] = sorted({f"{SV.stencil_name(r.stencil.kind)}:{r.schedule[0]}/"
+                                f"{r.schedule[1]}" for r in reqs3})
+    out["saturating_tuned" if alt else "saturating_default_schedules"] = sat2
+    print(f"[bench] serving saturating ({'tuned' if alt else 'default'} schedules): {sat2}",
+          file=sys.stderr, flush=True)
     return out
 
 
@@ -478,6 +504,22 @@ def main():
                             "ms_per_step": round(ms_s, 4)}
                 print(f"[bench] {name}: {by[name]}", file=sys.stderr, flush=True)
 
+    tuned = {}
+    if not args.no_tune:
+        # the autotuner's table at n^3 (FP64, ghost layer + refresh every
+        # sweep): every brick layout (8x8x8, 4x8x16 in the 4x16 and the 8x8
+        # fold) and every schedule of each, fastest first
+        cur.clear()
+        for sname in ("7pt", "13pt", "25pt", "125pt"):
+            st = S.Stencil.make(sname, seed=1)
+            t0 = time.perf_counter()
+            res = bk.tune(st, (n, n, n), bk.BK_F64, ghost=1, steps=10, rounds=2,
+                          warm_s=args.warm_s / 2, stream=stream)
+            tuned[sname] = {"best": bk.cand_name(res[0]),
+                            "ms_per_step": {bk.cand_name(c): round(c.ms, 4) for c in res},
+                            "tune_s": round(time.perf_counter() - t0, 2)}
+            print(f"[bench] tune {sname}: {tuned[sname]}", file=sys.stderr, flush=True)
+
     prec = {}
     if not args.no_precision:
         from bricklib_amd import precision as PR
@@ -550,6 +592,7 @@ def main():
         "valu_peak_measured": vpeak,
         "by_stencil": by,
         "precision": prec,
+        "tuner": tuned,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

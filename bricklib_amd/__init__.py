@@ -25,10 +25,11 @@ if _queues < 8:
     _os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 from ._lib import (BK_BF16, BK_COND7, BK_COND_REFERENCE, BK_COND_THRESHOLD, BK_CUBE125,
-                   BK_F32, BK_F64, BK_STAR7, BK_STAR13, BK_STAR25, BrickError, load)
+                   BK_F32, BK_F64, BK_LAYOUT_4x8x16, BK_LAYOUT_4x8x16_F88, BK_LAYOUT_8x8x8,
+                   BK_STAR7, BK_STAR13, BK_STAR25, BrickError, load)
 from .bricks import (Brick, BrickInfo, BrickStorage, DeviceBuffer, Event, PeriodicGrid,
                      Plan, Stream, plan_host, copyFromBrick, copyToBrick, device_count, device_sync, init_grid,
-                     movBrickInfo, movBrickStorage, sweep, sweeps)
+                     movBrickInfo, movBrickStorage, sweep, sweeps, tune, cand_name)
 from .stencils import Stencil
 from . import precision  # noqa: E402
 
@@ -38,5 +39,6 @@ __all__ = [
     "BrickInfo", "BrickStorage", "DeviceBuffer", "Event", "PeriodicGrid", "Plan", "Stream",
     "plan_host",
     "copyFromBrick", "copyToBrick", "device_count", "device_sync", "init_grid", "movBrickInfo",
-    "movBrickStorage", "sweep", "sweeps", "Stencil",
+    "movBrickStorage", "sweep", "sweeps", "Stencil", "tune", "cand_name", "BK_LAYOUT_8x8x8",
+    "BK_LAYOUT_4x8x16", "BK_LAYOUT_4x8x16_F88",
 ]

@@ -21,6 +21,9 @@ BK_F64, BK_F32, BK_BF16 = 0, 1, 2
 BK_STAR7, BK_STAR13, BK_STAR25, BK_CUBE125, BK_COND7 = 0, 1, 2, 3, 4
 BK_COND_REFERENCE, BK_COND_THRESHOLD = 0, 1
 BK_PLAN_COLUMN, BK_PLAN_QUAD = 1, 4
+BK_LAYOUT_8x8x8, BK_LAYOUT_4x8x16, BK_LAYOUT_4x8x16_F88 = 0, 1, 2
+BK_TUNE_ALL_LAYOUTS = 0x7
+BK_TUNE_BATCH_SCHEDULES = 0x100
 
 u32p = C.POINTER(C.c_uint32)
 dblp = C.POINTER(C.c_double)
@@ -43,6 +46,77 @@ class bk_sweep_desc(C.Structure):
         ("theta", C.c_double),
         ("cond_mode", C.c_int),
         ("plan", vp),
+        ("layout", C.c_int),
+        ("wg_bricks", C.c_int),
+    ]
+
+
+class bk_serve_config(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_double),
+        ("chunk", C.c_uint32),
+        ("max_active", C.c_uint32),
+        ("lanes", C.c_int),
+        ("use_graph", C.c_int),
+        ("ghost", C.c_int),
+        ("pipeline", C.c_int),
+        ("tune", C.c_int),
+        ("slab_bytes", C.c_size_t),
+    ]
+
+
+class bk_serve_req(C.Structure):
+    _fields_ = [
+        ("nb", C.c_long * 3),
+        ("stencil", C.c_int),
+        ("dtype", C.c_int),
+        ("steps", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("h_coeff", dblp),
+        ("theta", C.c_double),
+        ("cond_mode", C.c_int),
+        ("arrival", C.c_double),
+        ("keep", C.c_int),
+    ]
+
+
+class bk_serve_info(C.Structure):
+    _fields_ = [
+        ("state", C.c_int),
+        ("arrival", C.c_double),
+        ("admitted", C.c_double),
+        ("finished", C.c_double),
+        ("sweeps_done", C.c_uint32),
+        ("field_bytes", C.c_size_t),
+        ("d_result", vp),
+        ("plan_shape", C.c_int),
+        ("kcol", C.c_uint32),
+    ]
+
+
+class bk_serve_stats(C.Structure):
+    _fields_ = [
+        ("requests", C.c_uint32),
+        ("p50_s", C.c_double),
+        ("p99_s", C.c_double),
+        ("mean_s", C.c_double),
+        ("elapsed_s", C.c_double),
+        ("stencil_updates", C.c_double),
+        ("gstencil_s", C.c_double),
+        ("peak_hbm_bytes", C.c_double),
+        ("arena_reserved_bytes", C.c_double),
+        ("chunks", C.c_uint64),
+        ("tuned_classes", C.c_uint32),
+    ]
+
+
+class bk_tune_cand(C.Structure):
+    _fields_ = [
+        ("layout", C.c_int),
+        ("plan_shape", C.c_int),
+        ("kcol", C.c_uint32),
+        ("wg_bricks", C.c_int),
+        ("ms", C.c_float),
     ]
 
 
@@ -112,6 +186,28 @@ SIGNATURES = {
     "bk_event_sync": (C.c_int, [vp]),
     "bk_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
     "bk_init_grid": (C.c_int, [longp, C.c_int, u32p, u32p]),
+    "bk_layout_extents": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
+    "bk_fill_uniform_layout": (C.c_int, [C.c_int, C.c_int, vp, C.c_size_t, longp, C.c_int,
+                                         C.c_uint64, C.c_double, C.c_double, vp]),
+    "bk_tiled_list": (C.c_int, [longp, C.c_int, C.c_uint32, C.c_uint32, u32p, u32p]),
+    "bk_tune_sweep": (C.c_int, [C.c_int, C.c_int, dblp, C.c_double, C.c_int, longp, C.c_int,
+                                C.c_int, C.c_int, C.c_double, C.c_uint, C.POINTER(bk_tune_cand),
+                                C.c_uint32, u32p, vp]),
+    "bk_tune_lookup": (C.c_int, [C.c_int, C.c_int, longp, C.c_int, C.c_uint,
+                                 C.POINTER(bk_tune_cand)]),
+    "bk_server_create": (C.c_int, [C.POINTER(bk_serve_config), C.POINTER(vp)]),
+    "bk_server_destroy": (C.c_int, [vp]),
+    "bk_server_submit": (C.c_int, [vp, C.POINTER(bk_serve_req), u32p]),
+    "bk_server_prewarm": (C.c_int, [vp, C.POINTER(bk_serve_req), C.c_uint32]),
+    "bk_server_run": (C.c_int, [vp, C.c_int]),
+    "bk_server_start": (C.c_int, [vp]),
+    "bk_server_stop": (C.c_int, [vp]),
+    "bk_server_wait": (C.c_int, [vp, C.c_uint32, C.c_double]),
+    "bk_server_info": (C.c_int, [vp, C.c_uint32, C.POINTER(bk_serve_info)]),
+    "bk_server_result": (C.c_int, [vp, C.c_uint32, vp, C.c_size_t]),
+    "bk_server_release": (C.c_int, [vp, C.c_uint32]),
+    "bk_server_stats": (C.c_int, [vp, C.POINTER(bk_serve_stats)]),
+    "bk_server_reset_stats": (C.c_int, [vp]),
     "bk_ghost_pairs": (C.c_int, [longp, u32p, u32p]),
     "bk_interior_list": (C.c_int, [longp, u32p, u32p]),
     "bk_sweep": (C.c_int, [C.POINTER(bk_sweep_desc), vp]),

@@ -210,8 +210,11 @@ class Request:
     done: int = 0
     admitted: float = 0.0
     finished: float = 0.0
+    schedule: tuple = (0, 0)  # (plan shape, kcol) a server ran it by
     _coeff: np.ndarray = field(default=None, repr=False)
     _rec: bytes = field(default=None, repr=False)
+    _server: object = field(default=None, repr=False)  # serve.Server that ran it
+    _sid: int = field(default=-1, repr=False)
 
     @property
     def cells(self) -> int:
@@ -249,6 +252,9 @@ class Request:
         self._rec = bytes(self.req_struct(0))
 
     def release(self) -> None:
+        if self._server is not None:
+            check(load().bk_server_release(_vp(self._server.handle), self._sid))
+            return
         for f in (self.f0, self.f1):
             if f is not None:
                 f.free()
@@ -258,19 +264,44 @@ class Request:
     @property
     def result_ptr(self) -> int:
         """The field holding the latest result."""
+        if self._server is not None:
+            info = _lib.bk_serve_info()
+            check(load().bk_server_info(_vp(self._server.handle), self._sid, C.byref(info)))
+            if not info.d_result:
+                raise ValueError(f"request {self.rid}: no kept result on the server")
+            return info.d_result
         return (self.f1 if self.parity else self.f0).ptr
 
     def result_host(self, stream=None) -> np.ndarray:
         from .bricks import DTYPE_NP
         out = np.empty(self.field_bytes // DTYPE_SIZE[self.dtype], DTYPE_NP[self.dtype])
-        check(load().bk_memcpy_d2h(out.ctypes.data_as(C.c_void_p), _vp(self.result_ptr),
-                                   C.c_size_t(out.nbytes), _vp(_stream_of(stream))))
+        if self._server is not None:
+            check(load().bk_server_result(_vp(self._server.handle), self._sid,
+                                          out.ctypes.data_as(C.c_void_p),
+                                          C.c_size_t(out.nbytes)))
+        else:
+            check(load().bk_memcpy_d2h(out.ctypes.data_as(C.c_void_p), _vp(self.result_ptr),
+                                       C.c_size_t(out.nbytes), _vp(_stream_of(stream))))
         return out.reshape(-1, 512)
+
+    def interior_index(self) -> np.ndarray:
+        """Ids of the interior bricks in grid order (k slowest) for this
+        request's layout (ghost 1: inside the ghost layer; 0: every brick)."""
+        if self.shape is not None:
+            return self.shape.grid.interior
+        if not self.ghost:
+            return np.arange(int(np.prod(self.nb)), dtype=np.uint32)
+        n = C.c_uint32()
+        check(load().bk_interior_list(_lib.longs(self.nb), None, C.byref(n)))
+        out = np.empty(n.value, np.uint32)
+        check(load().bk_interior_list(_lib.longs(self.nb), out.ctypes.data_as(_lib.u32p),
+                                      C.byref(n)))
+        return out
 
     def interior_host(self, stream=None) -> np.ndarray:
         """The latest result's interior bricks in grid order (k slowest), the
         same array for either layout."""
-        return self.result_host(stream)[self.shape.grid.interior]
+        return self.result_host(stream)[self.interior_index()]
 
     def req_struct(self, ticks: int) -> bk_batch_req:
         g = self.shape.grid

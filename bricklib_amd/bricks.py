@@ -341,12 +341,22 @@ class PeriodicGrid:
     ghost=0: an nb^3 brick grid whose adjacency wraps (init_grid with
       periodic=True); every brick is interior and nothing needs refreshing.
       A sweep reads the periodic image itself where the ghost layout reads
-      its copy, so both give the same values, bit for bit."""
+      its copy, so both give the same values, bit for bit.
 
-    def __init__(self, nb, stream=None, ghost: int = 1, tile: int = 32):
+    layout: the brick shape and vector fold of the grid's fields
+    (BK_LAYOUT_8x8x8, BK_LAYOUT_4x8x16 or BK_LAYOUT_4x8x16_F88); nb counts
+    bricks of that shape per axis (i, j, k), so the interior holds
+    nb[0]*BI x nb[1]*BJ x nb[2]*BK cells.  Each cell gets the same value in
+    every layout."""
+
+    def __init__(self, nb, stream=None, ghost: int = 1, tile: int = 32, layout: int = 0):
         self.nb = tuple(int(x) for x in nb)
         if ghost not in (0, 1):
             raise ValueError("ghost must be 0 or 1")
+        ext = (C.c_int * 3)()
+        check(load().bk_layout_extents(int(layout), ext))
+        self.brick_layout = int(layout)
+        self.extents = tuple(int(x) for x in ext)  # (BK, BJ, BI)
         self.ghost = int(ghost)
         self.padded = tuple(x + 2 * self.ghost for x in self.nb)
         self.grid, self.info = init_grid(self.padded, periodic=not self.ghost)
@@ -381,27 +391,27 @@ class PeriodicGrid:
         self.d_grid = DeviceBuffer.from_array(self.grid.reshape(-1), stream)
 
         self._plans: dict = {}
+        self._tuned: dict = {}
 
     def tiled_order(self, T) -> np.ndarray:
-        """Interior bricks tile by tile: Ti x Tj bricks in (i, j) (T an int
-        for square tiles or a pair (Ti, Tj)), all k."""
+        """Interior bricks tile by tile (bk_tiled_list): Ti x Tj bricks in
+        (i, j) (T an int for square tiles or a pair (Ti, Tj)), all k."""
         Ti, Tj = (int(T[0]), int(T[1])) if isinstance(T, (tuple, list)) else (int(T), int(T))
         if Ti < 1 or Tj < 1:
             raise ValueError("tile sides must be positive")
-        nb0, nb1, nb2 = self.nb
-        g = self.grid  # (k, j, i) padded numbering
-        o = self.ghost
-        out = []
-        for j0 in range(0, nb1, Tj):
-            for i0 in range(0, nb0, Ti):
-                blk = g[o:o + nb2, o + j0:o + min(j0 + Tj, nb1), o + i0:o + min(i0 + Ti, nb0)]
-                out.append(blk.reshape(-1))
-        return np.ascontiguousarray(np.concatenate(out), dtype=np.uint32)
+        n = C.c_uint32()
+        check(load().bk_tiled_list(longs(self.nb), self.ghost, Ti, Tj, None, C.byref(n)))
+        out = np.empty(n.value, dtype=np.uint32)
+        check(load().bk_tiled_list(longs(self.nb), self.ghost, Ti, Tj,
+                                   out.ctypes.data_as(_lib.u32p), C.byref(n)))
+        return out
 
     def plan(self, shape: str = "column", kcol: int = 16, every: bool = False) -> Plan:
         """A cached plan over the interior bricks in sweep order (every=True:
         over every brick of the padded grid, ghost bricks included)."""
         key = (shape, int(kcol), bool(every))
+        if self.brick_layout != 0:
+            raise ValueError("plans are built for the 8x8x8 layout only")
         if key not in self._plans:
             blist = np.arange(self.nbricks, dtype=np.uint32) if every else self.sweep_list
             self._plans[key] = Plan(self.info, blist, shape, kcol)
@@ -430,67 +440,51 @@ class PeriodicGrid:
         return self._d_every
 
     def auto_plan(self, stencil_kind: int):
-        """Default schedule per stencil, from warmed measurements at 512^3
-        on MI355X: 2x2 column quads of 4 levels for the cube (1-3% ahead
-        of 8 levels, whose smaller halo share is outweighed by the shorter
-        tail of more, shorter units), k-marching columns of 8 bricks for
-        the radius-4 star, and one brick per wave (no plan) for the
-        radius-1 and radius-2 stars."""
+        """Default schedule per stencil: the tuned one when tune() ran for
+        it on this grid, else, from warmed measurements at 512^3 on MI355X,
+        2x2 column quads of 4 levels for the cube (1-3% ahead of 8 levels,
+        whose smaller halo share is outweighed by the shorter tail of more,
+        shorter units), k-marching columns of 8 bricks for the radius-4
+        star, and one brick per wave (no plan) for the radius-1 and radius-2
+        stars.  The 4x8x16 layouts always run one brick per wave."""
         from ._lib import BK_CUBE125, BK_STAR25
-        tuned = getattr(self, "_tuned", {}).get(int(stencil_kind))
+        tuned = self._tuned.get(int(stencil_kind))
         if tuned is not None:
-            return None if tuned == "none" else self.plan(*tuned)
+            return None if tuned.plan_shape == 0 else self.plan(
+                "quad" if tuned.plan_shape == _lib.BK_PLAN_QUAD else "column", tuned.kcol)
+        if self.brick_layout != 0:
+            return None
         if stencil_kind == BK_CUBE125:
             return self.plan("quad", 4)
         if stencil_kind == BK_STAR25:
             return self.plan("column", 8)
         return None
 
-    def tune(self, st, dtype: int = BK_F64, steps: int = 5, stream=None,
-             candidates=None, rounds: int = 3, warm_s: float = 0.3) -> dict:
-        """Time every candidate schedule of stencil `st` on this grid and
-        make the fastest this grid's choice for plan="auto".  All schedules
-        compute the same bits, so the choice only changes the speed.
-        Candidates: no plan and column plans of 4, 8 and 16 bricks for the
-        stars; no plan and quad plans of 4, 8 and 16 levels for the cube.
-        The candidates first run untimed for warm_s seconds (the GPU's clock
-        rises over the first fraction of a second of sustained work, which
-        would favour whichever candidate is timed last), then each is timed
-        over `steps` sweeps without refresh in `rounds` interleaved rounds,
-        keeping its best.  Returns {"best": ..., "ms": {candidate: ms per
-        sweep}}."""
-        import time
-        from ._lib import BK_CUBE125
-        if candidates is None:
-            shape = "quad" if st.kind == BK_CUBE125 else "column"
-            candidates = ["none"] + [(shape, k) for k in (4, 8, 16)]
-        a = Brick(self.info, self.field_storage(dtype), 0)
-        b = Brick(self.info, self.field_storage(dtype), 0)
-        self.fill_uniform(a, 99, stream=stream)
-        s = stream if stream is not None else Stream()
-        plans = {c: None if c == "none" else self.plan(*c) for c in candidates}
-        for pl in plans.values():
-            sweeps(st, a, b, self, 1, refresh=False, stream=s, plan=pl)
-        s.sync()
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < warm_s:
-            for pl in plans.values():
-                sweeps(st, a, b, self, steps, refresh=False, stream=s, plan=pl)
-            s.sync()
-        times = {c: float("inf") for c in candidates}
-        e0, e1 = Event(), Event()
-        for _ in range(max(1, rounds)):
-            for cand, pl in plans.items():
-                e0.record(s)
-                sweeps(st, a, b, self, steps, refresh=False, stream=s, plan=pl)
-                e1.record(s)
-                e1.sync()
-                times[cand] = min(times[cand], e0.elapsed_ms(e1) / steps)
-        best = min(times, key=times.get)
-        if not hasattr(self, "_tuned"):
-            self._tuned = {}
-        self._tuned[int(st.kind)] = best
-        return {"best": best, "ms": {str(k): round(v, 4) for k, v in times.items()}}
+    def auto_wg_bricks(self, stencil_kind: int) -> int:
+        """Bricks per workgroup of the per-brick kernels: the tuned value,
+        else the kernels' default (0)."""
+        tuned = self._tuned.get(int(stencil_kind))
+        return tuned.wg_bricks if tuned is not None else 0
+
+    @property
+    def cell_extents(self) -> tuple:
+        """Interior cells per axis (i, j, k)."""
+        BK, BJ, BI = self.extents
+        return (self.nb[0] * BI, self.nb[1] * BJ, self.nb[2] * BK)
+
+    def tune(self, st, dtype: int = BK_F64, steps: int = 5, stream=None, rounds: int = 3,
+             warm_s: float = 0.3) -> dict:
+        """Time every schedule of this grid's layout for stencil `st`
+        (bk_tune_sweep on a grid of the same cells and ghost layer: per
+        brick with 1/2/4/8 bricks per workgroup, k-columns or column quads)
+        and make the fastest this grid's choice for plan="auto".  All
+        schedules compute the same bits.  Returns {"best": candidate,
+        "ms": {candidate: ms per sweep}} with candidates named by
+        cand_name()."""
+        res = tune(st, self.cell_extents, dtype, ghost=self.ghost, steps=steps, rounds=rounds,
+                   warm_s=warm_s, layouts=(self.brick_layout,), stream=stream)
+        self._tuned[int(st.kind)] = res[0]
+        return {"best": cand_name(res[0]), "ms": {cand_name(c): round(c.ms, 4) for c in res}}
 
     @property
     def nbricks(self) -> int:
@@ -511,9 +505,9 @@ class PeriodicGrid:
 
     def fill_uniform(self, brick: Brick, seed: int, lo: float = -1.0, hi: float = 1.0,
                      stream=None) -> None:
-        check(load().bk_fill_uniform_grid(brick.dtype, _vp(brick.ptr), C.c_size_t(brick.step),
-                                          longs(self.nb), self.ghost, C.c_uint64(seed), lo, hi,
-                                          _vp(_stream_of(stream))))
+        check(load().bk_fill_uniform_layout(self.brick_layout, brick.dtype, _vp(brick.ptr),
+                                            C.c_size_t(brick.step), longs(self.nb), self.ghost,
+                                            C.c_uint64(seed), lo, hi, _vp(_stream_of(stream))))
 
     def ghost_refresh(self, brick: Brick, stream=None) -> None:
         if not self.npairs:
@@ -556,7 +550,7 @@ def _resolve_plan(plan, st: Stencil, grid: PeriodicGrid):
 
 def _desc(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, coeff_keep: list,
           blist: DeviceBuffer | None = None, nlist: int | None = None,
-          plan=None) -> bk_sweep_desc:
+          plan=None, wg_bricks=None) -> bk_sweep_desc:
     if out.dtype != inp.dtype:
         raise ValueError("input and output fields must share a dtype")
     coeff = np.ascontiguousarray(st.coeff, dtype=np.float64)
@@ -580,22 +574,28 @@ def _desc(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, coeff_keep: l
     pl = _resolve_plan(plan, st, grid)
     d.plan = pl.handle if pl is not None else None
     coeff_keep.append(pl)
+    d.layout = getattr(grid, "brick_layout", 0)
+    if wg_bricks is None:
+        wg_bricks = (grid.auto_wg_bricks(st.kind)
+                     if plan == "auto" and hasattr(grid, "auto_wg_bricks") else 0)
+    d.wg_bricks = 0 if pl is not None else int(wg_bricks)
     return d
 
 
 def sweep(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, stream=None,
-          plan="auto") -> None:
+          plan="auto", wg_bricks=None) -> None:
     """One sweep over the interior bricks: out = stencil(in).  ``plan`` is
     "auto", None (plain brick list, one brick per wave), "column", "quad"
-    or a Plan."""
+    or a Plan; ``wg_bricks`` the bricks per workgroup of the per-brick
+    kernels (None: tuned or default)."""
     keep: list = []
-    d = _desc(st, out, inp, grid, keep, plan=plan)
+    d = _desc(st, out, inp, grid, keep, plan=plan, wg_bricks=wg_bricks)
     check(load().bk_sweep(C.byref(d), _vp(_stream_of(stream))))
 
 
 def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
            use_graph: bool = False, refresh: bool = True, stream=None, plan="auto",
-           ghost_iters: int = 1, fused: bool = False) -> Brick:
+           ghost_iters: int = 1, fused: bool = False, wg_bricks=None) -> Brick:
     """``nsweeps`` ping-pong sweeps a->b->a..., each preceded by the periodic
     ghost refresh of its input -- or, with ghost_iters > 1, the refresh once
     per ghost_iters sweeps and the sweeps in between over every brick of the
@@ -604,7 +604,7 @@ def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
     wave).  Same result in every form.  Returns the field holding the
     result."""
     keep: list = []
-    d = _desc(st, b, a, grid, keep, plan=plan)
+    d = _desc(st, b, a, grid, keep, plan=plan, wg_bricks=wg_bricks)
     pairs = grid.d_pairs.ptr if (refresh and grid.d_pairs is not None) else None
     npairs = grid.npairs if refresh else 0
     if fused:
@@ -632,8 +632,39 @@ def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
     return b if nsweeps % 2 == 1 else a
 
 
+def cand_name(c) -> str:
+    """A tuner candidate as text: layout, then the schedule."""
+    lay = {0: "8x8x8", 1: "4x8x16", 2: "4x8x16/f88"}[c.layout]
+    if c.plan_shape == 0:
+        return f"{lay} brick wg{c.wg_bricks}"
+    shape = "quad" if c.plan_shape == _lib.BK_PLAN_QUAD else "column"
+    return f"{lay} {shape}{c.kcol}"
+
+
+def tune(st: Stencil, cells, dtype: int = BK_F64, ghost: int = 1, steps: int = 5,
+         rounds: int = 3, warm_s: float = 0.3, layouts=(0, 1, 2), batch_only: bool = False,
+         stream=None) -> list:
+    """bk_tune_sweep: time every (layout, schedule) candidate for stencil
+    `st` on a periodic grid of `cells` (i, j, k) and return them fastest
+    first (bk_tune_cand records: layout, plan_shape, kcol, wg_bricks, ms)."""
+    flags = 0
+    for lay in layouts:
+        flags |= 1 << int(lay)
+    if batch_only:
+        flags |= _lib.BK_TUNE_BATCH_SCHEDULES
+    coeff = np.ascontiguousarray(st.coeff, dtype=np.float64)
+    n = C.c_uint32()
+    args = (int(st.kind), int(dtype), coeff.ctypes.data_as(_lib.dblp), float(st.theta),
+            int(st.cond_mode), longs(cells), int(ghost), int(steps), int(rounds), float(warm_s),
+            C.c_uint(flags))
+    check(load().bk_tune_sweep(*args, None, 0, C.byref(n), None))
+    out = (_lib.bk_tune_cand * n.value)()
+    check(load().bk_tune_sweep(*args, out, n.value, C.byref(n), _vp(_stream_of(stream))))
+    return list(out[:n.value])
+
+
 __all__ = [
-    "DeviceBuffer", "Stream", "Event", "BrickInfo", "init_grid", "movBrickInfo",
+    "tune", "cand_name", "DeviceBuffer", "Stream", "Event", "BrickInfo", "init_grid", "movBrickInfo",
     "BrickStorage", "movBrickStorage", "Brick", "copyToBrick", "copyFromBrick",
     "PeriodicGrid", "Plan", "plan_host", "sweep", "sweeps", "device_count", "device_sync", "DTYPE_NP",
     "DTYPE_SIZE", "BK_F64", "BK_F32", "BK_BF16",

@@ -170,6 +170,11 @@ __global__ void zero1(unsigned char *p, size_t n) {
 // seeded uniform fill of the interior bricks (SplitMix64 finaliser of
 // seed + (n+1)*golden, 53-bit uniform), same values as the oracle
 // ---------------------------------------------------------------------------
+// Brick extents (k, j, i) and vector fold (j, i) of a layout.
+struct FoldGeom {
+  uint32_t bk, bj, bi, fj, fi;
+};
+
 __device__ __forceinline__ uint64_t hash64(uint64_t seed, uint64_t n) {
   uint64_t z = seed + (n + 1) * 0x9E3779B97F4A7C15ULL;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -181,7 +186,8 @@ template <class T>
 __global__ __launch_bounds__(256) void fill_uniform(T *__restrict__ dat, uint64_t step,
                                                     uint32_t nb0, uint32_t nb1,
                                                     uint32_t nb2, uint32_t ghost,
-                                                    uint64_t seed, double lo, double hi) {
+                                                    FoldGeom g, uint64_t seed, double lo,
+                                                    double hi) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nbricks = (uint64_t)nb0 * nb1 * nb2;
   if (t >= nbricks * 512) return;
@@ -189,9 +195,15 @@ __global__ __launch_bounds__(256) void fill_uniform(T *__restrict__ dat, uint64_
   const uint32_t e = (uint32_t)(t & 511);
   const uint32_t ti = (uint32_t)(bi % nb0), tj = (uint32_t)((bi / nb0) % nb1),
                  tk = (uint32_t)(bi / ((uint64_t)nb0 * nb1));
-  const uint32_t k = e >> 6, j = (e >> 3) & 7, i = e & 7;
-  const uint64_t n0 = (uint64_t)nb0 * 8, n1 = (uint64_t)nb1 * 8;
-  const uint64_t x = ti * 8 + i, y = tj * 8 + j, z = tk * 8 + k;
+  // element e -> (k, j, i) through the fold: vector e / 64, lane e % 64
+  const uint32_t vec = e >> 6, lane = e & 63;
+  const uint32_t nj = g.bj / g.fj, ni = g.bi / g.fi;
+  const uint32_t k = vec / (nj * ni);
+  const uint32_t j = ((vec / ni) % nj) * g.fj + lane / g.fi;
+  const uint32_t i = (vec % ni) * g.fi + lane % g.fi;
+  const uint64_t n0 = (uint64_t)nb0 * g.bi, n1 = (uint64_t)nb1 * g.bj;
+  const uint64_t x = (uint64_t)ti * g.bi + i, y = (uint64_t)tj * g.bj + j,
+                 z = (uint64_t)tk * g.bk + k;
   const double u = (double)(hash64(seed, x + n0 * (y + n1 * z)) >> 11) * 0x1.0p-53;
   const double v = lo + (hi - lo) * u;
   const uint64_t S0 = nb0 + 2 * ghost, S1 = nb1 + 2 * ghost;
@@ -281,8 +293,8 @@ __global__ __launch_bounds__(256) void compare_pass1(int dta, const void *a, uin
 }
 
 template <int NV>
-__global__ __launch_bounds__(256) void reduce_pass2(double *part, int nblocks, int m0, int m1,
-                                                    int m2, int m3, int m4) {
+__global__ __launch_bounds__(256) void reduce_pass2(const double *part, int nblocks, double *out,
+                                                    int m0, int m1, int m2, int m3, int m4) {
   double v[NV];
   const int ms[5] = {m0, m1, m2, m3, m4};
   bool mx[NV];
@@ -299,7 +311,7 @@ __global__ __launch_bounds__(256) void reduce_pass2(double *part, int nblocks, i
     }
   }
   __syncthreads();
-  block_reduce<NV>(v, mx, part);  // result into part[0..NV)
+  block_reduce<NV>(v, mx, out);  // out may be part itself: every read precedes the write
 }
 
 // ---------------------------------------------------------------------------
@@ -436,26 +448,34 @@ hipError_t launch_zero(void *dst, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_fill_uniform(int dtype, void *dat, size_t step, const long nb[3], int ghost,
-                               uint64_t seed, double lo, double hi, hipStream_t s) {
+hipError_t launch_fill_uniform(int layout, int dtype, void *dat, size_t step, const long nb[3],
+                               int ghost, uint64_t seed, double lo, double hi, hipStream_t s) {
   const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2] * 512;
   if (n == 0) return hipSuccess;
+  FoldGeom g;
+  switch (layout) {
+    case BK_LAYOUT_8x8x8: g = {8, 8, 8, 8, 8}; break;
+    case BK_LAYOUT_4x8x16: g = {4, 8, 16, 4, 16}; break;
+    case BK_LAYOUT_4x8x16_F88: g = {4, 8, 16, 8, 8}; break;
+    default: return hipErrorInvalidValue;
+  }
   const uint32_t nblk = (uint32_t)((n + 255) / 256);
+  const uint32_t n0 = (uint32_t)nb[0], n1 = (uint32_t)nb[1], n2 = (uint32_t)nb[2];
   switch (dtype) {
     case BK_F64:
       hipLaunchKernelGGL(fill_uniform<double>, dim3(nblk), dim3(256), 0, s,
-                         static_cast<double *>(dat), (uint64_t)step, (uint32_t)nb[0],
-                         (uint32_t)nb[1], (uint32_t)nb[2], (uint32_t)ghost, seed, lo, hi);
+                         static_cast<double *>(dat), (uint64_t)step, n0, n1, n2,
+                         (uint32_t)ghost, g, seed, lo, hi);
       break;
     case BK_F32:
       hipLaunchKernelGGL(fill_uniform<float>, dim3(nblk), dim3(256), 0, s,
-                         static_cast<float *>(dat), (uint64_t)step, (uint32_t)nb[0],
-                         (uint32_t)nb[1], (uint32_t)nb[2], (uint32_t)ghost, seed, lo, hi);
+                         static_cast<float *>(dat), (uint64_t)step, n0, n1, n2,
+                         (uint32_t)ghost, g, seed, lo, hi);
       break;
     case BK_BF16:
       hipLaunchKernelGGL(fill_uniform<bf16>, dim3(nblk), dim3(256), 0, s,
-                         static_cast<bf16 *>(dat), (uint64_t)step, (uint32_t)nb[0],
-                         (uint32_t)nb[1], (uint32_t)nb[2], (uint32_t)ghost, seed, lo, hi);
+                         static_cast<bf16 *>(dat), (uint64_t)step, n0, n1, n2,
+                         (uint32_t)ghost, g, seed, lo, hi);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -463,12 +483,12 @@ hipError_t launch_fill_uniform(int dtype, void *dat, size_t step, const long nb[
 }
 
 hipError_t launch_field_stats(int dtype, const void *dat, size_t step, const uint32_t *list,
-                              uint32_t nlist, size_t brick_elems, double *scratch,
+                              uint32_t nlist, size_t brick_elems, double *scratch, double *out,
                               hipStream_t s) {
   hipLaunchKernelGGL(stats_pass1, dim3(kReduceBlocks), dim3(256), 0, s, dtype, dat,
                      (uint64_t)step, list, nlist, (uint32_t)brick_elems, scratch);
-  hipLaunchKernelGGL(reduce_pass2<3>, dim3(1), dim3(256), 0, s, scratch, kReduceBlocks, 0, 0, 1,
-                     0, 0);
+  hipLaunchKernelGGL(reduce_pass2<3>, dim3(1), dim3(256), 0, s, scratch, kReduceBlocks,
+                     out ? out : scratch, 0, 0, 1, 0, 0);
   return hipGetLastError();
 }
 
@@ -479,8 +499,8 @@ hipError_t launch_field_compare(int dta, const void *a, size_t step_a, int dtr, 
   hipLaunchKernelGGL(compare_pass1, dim3(kReduceBlocks), dim3(256), 0, s, dta, a,
                      (uint64_t)step_a, dtr, r, (uint64_t)step_r, list, nlist,
                      (uint32_t)brick_elems, floor_, atol, rtol, scratch);
-  hipLaunchKernelGGL(reduce_pass2<5>, dim3(1), dim3(256), 0, s, scratch, kReduceBlocks, 1, 1, 0,
-                     0, 0);
+  hipLaunchKernelGGL(reduce_pass2<5>, dim3(1), dim3(256), 0, s, scratch, kReduceBlocks, scratch,
+                     1, 1, 0, 0, 0);
   return hipGetLastError();
 }
 

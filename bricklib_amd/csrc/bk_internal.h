@@ -28,6 +28,8 @@ struct SweepLaunch {
   // are img[img_off[b] .. img_off[b+1]); each also receives b's output
   const uint32_t *img_off = nullptr;
   const uint32_t *img = nullptr;
+  int layout = BK_LAYOUT_8x8x8;  // brick shape and fold of both fields
+  int wg_bricks = 0;             // bricks per workgroup (0: the kernel's default)
 };
 
 // Device view of an execution plan (see bk_plan.hip).
@@ -54,10 +56,14 @@ inline bool quad_offsets_fit(uint32_t nbricks, uint64_t step) {
 
 // Per-brick kernels over L.list (any list).
 hipError_t launch_star(const SweepLaunch &L, hipStream_t s);
+// Per-brick kernels for the 4x8x16 layouts (any stencil).
+hipError_t launch_shape(const SweepLaunch &L, hipStream_t s);
 hipError_t launch_cube(const SweepLaunch &L, hipStream_t s);
 // Marching kernels over a plan: stars over k-columns, the cube over 2x2
 // column quads streamed plane by plane.
 hipError_t launch_star_columns(const SweepLaunch &L, const PlanView &P, hipStream_t s);
+// Stars over 2x2 column quads: four waves share one LDS tile of the block.
+hipError_t launch_star_quads(const SweepLaunch &L, const PlanView &P, hipStream_t s);
 hipError_t launch_cube_quads(const SweepLaunch &L, const PlanView &P, hipStream_t s);
 
 int plan_fail(int code, const char *msg);
@@ -109,15 +115,16 @@ hipError_t launch_zero(void *dst, size_t bytes, hipStream_t s);
 hipError_t launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_valu_probe(int dtype, int chains, uint32_t nblocks, uint32_t iters, void *out,
                              hipStream_t s);
-hipError_t launch_fill_uniform(int dtype, void *dat, size_t step,
+hipError_t launch_fill_uniform(int layout, int dtype, void *dat, size_t step,
                                const long nb[3], int ghost, uint64_t seed,
                                double lo, double hi, hipStream_t s);
-// Deterministic two-pass FP64 reductions; `scratch` holds kReduceBlocks*8
-// doubles.  Results land in scratch[0 .. nout).
+// Deterministic two-pass FP64 reductions; `scratch` holds kReduceBlocks*nout
+// doubles (pass-1 partials).  Results land in scratch[0 .. nout), or in
+// out[0 .. nout) for launch_field_stats when `out` is not null.
 constexpr int kReduceBlocks = 1024;
 hipError_t launch_field_stats(int dtype, const void *dat, size_t step,
                               const uint32_t *list, uint32_t nlist,
-                              size_t brick_elems, double *scratch,
+                              size_t brick_elems, double *scratch, double *out,
                               hipStream_t s);
 hipError_t launch_field_compare(int dta, const void *a, size_t step_a,
                                 int dtr, const void *r, size_t step_r,

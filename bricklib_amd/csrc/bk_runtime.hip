@@ -46,6 +46,17 @@ inline hipStream_t S(bk_stream s) { return static_cast<hipStream_t>(s); }
 
 bool valid_dtype(int dt) { return dt == BK_F64 || dt == BK_F32 || dt == BK_BF16; }
 
+bool valid_layout(int l) {
+  return l == BK_LAYOUT_8x8x8 || l == BK_LAYOUT_4x8x16 || l == BK_LAYOUT_4x8x16_F88;
+}
+
+// brick extents (k, j, i) of a layout
+void layout_ext(int l, int e[3]) {
+  e[0] = l == BK_LAYOUT_8x8x8 ? 8 : 4;
+  e[1] = 8;
+  e[2] = l == BK_LAYOUT_8x8x8 ? 8 : 16;
+}
+
 int dt_size(int dt) { return dt == BK_F64 ? 8 : dt == BK_F32 ? 4 : dt == BK_BF16 ? 2 : 0; }
 
 // Per-device FP64 reduction scratch, allocated once and guarded by a mutex;
@@ -54,7 +65,7 @@ struct Scratch {
   std::mutex mu;
   std::map<int, double *> buf;
 };
-Scratch &scratch() {
+Scratch &reduce_scratch() {
   static Scratch s;
   return s;
 }
@@ -62,7 +73,7 @@ Scratch &scratch() {
 int get_scratch(double **p) {
   int dev = 0;
   BK_HIP(hipGetDevice(&dev));
-  auto &sc = scratch();
+  auto &sc = reduce_scratch();
   auto it = sc.buf.find(dev);
   if (it != sc.buf.end()) {
     *p = it->second;
@@ -102,6 +113,16 @@ int check_desc(const bk_sweep_desc *d, bool pingpong = false) {
   if (d->stencil == BK_COND7 && d->cond_mode != BK_COND_REFERENCE &&
       d->cond_mode != BK_COND_THRESHOLD)
     return fail(BK_ERR_ARG, "bk_sweep: unknown cond mode %d", d->cond_mode);
+  if (!valid_layout(d->layout)) return fail(BK_ERR_ARG, "bk_sweep: unknown layout %d", d->layout);
+  if (d->wg_bricks != 0 && d->wg_bricks != 1 && d->wg_bricks != 2 && d->wg_bricks != 4 &&
+      d->wg_bricks != 8)
+    return fail(BK_ERR_ARG, "bk_sweep: wg_bricks must be 0, 1, 2, 4 or 8 (got %d)",
+                d->wg_bricks);
+  if (d->wg_bricks == 8 && (d->layout != BK_LAYOUT_8x8x8 || d->stencil == BK_CUBE125))
+    return fail(BK_ERR_UNSUPPORTED, "bk_sweep: 8 bricks per workgroup is built for the 8x8x8 "
+                "stars only");
+  if (d->layout != BK_LAYOUT_8x8x8 && d->plan)
+    return fail(BK_ERR_UNSUPPORTED, "bk_sweep: plans are built for the 8x8x8 layout only");
   if (!d->plan && d->nlist == 0) return BK_OK;
   if (!d->d_adj || !d->d_in || !d->d_out || (!d->plan && !d->d_list))
     return fail(BK_ERR_ARG, "bk_sweep: null device pointer");
@@ -143,6 +164,8 @@ void to_launch(const bk_sweep_desc *d, bk::SweepLaunch &L) {
   for (int q = 0; q < 125; ++q) L.coeff[q] = q < L.ncoeff ? d->h_coeff[q] : 0.0;
   L.theta = d->theta;
   L.cond_mode = d->cond_mode;
+  L.layout = d->layout;
+  L.wg_bricks = d->wg_bricks;
 }
 
 hipError_t launch_sweep(const bk::SweepLaunch &L, const bk_plan_s *plan, hipStream_t s) {
@@ -152,12 +175,14 @@ hipError_t launch_sweep(const bk::SweepLaunch &L, const bk_plan_s *plan, hipStre
     if (L.stencil == BK_CUBE125 && P.shape == BK_PLAN_QUAD) return bk::launch_cube_quads(L, P, s);
     if (L.stencil != BK_CUBE125 && P.shape == BK_PLAN_COLUMN)
       return bk::launch_star_columns(L, P, s);
+    if (L.stencil != BK_CUBE125 && P.shape == BK_PLAN_QUAD) return bk::launch_star_quads(L, P, s);
     bk::SweepLaunch F = L;  // other pairings: per-brick kernel over the plan's bricks
     F.list = P.flat;
     F.nlist = P.ncovered;
     return F.stencil == BK_CUBE125 ? bk::launch_cube(F, s) : bk::launch_star(F, s);
   }
   if (L.nlist == 0) return hipSuccess;
+  if (L.layout != BK_LAYOUT_8x8x8) return bk::launch_shape(L, s);
   if (L.stencil == BK_CUBE125) return bk::launch_cube(L, s);
   return bk::launch_star(L, s);
 }
@@ -232,6 +257,13 @@ int bk_stencil_ncoeff(int stencil) {
     case BK_CUBE125: return 125;
     default: return 0;
   }
+}
+
+int bk_layout_extents(int layout, int ext[3]) {
+  if (!ext) return fail(BK_ERR_ARG, "bk_layout_extents: null pointer");
+  if (!valid_layout(layout)) return fail(BK_ERR_ARG, "bk_layout_extents: unknown layout %d", layout);
+  layout_ext(layout, ext);
+  return BK_OK;
 }
 
 int bk_stencil_radius(int stencil) {
@@ -633,6 +665,8 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
   put(d->h_coeff, sizeof(double) * (size_t)bk_stencil_ncoeff(d->stencil));
   put(&d->theta, sizeof(d->theta));
   put(&d->cond_mode, sizeof(d->cond_mode));
+  put(&d->layout, sizeof(d->layout));
+  put(&d->wg_bricks, sizeof(d->wg_bricks));
   put(&d_pairs, sizeof(d_pairs));
   put(&npairs, sizeof(npairs));
   put(&nsweeps, sizeof(nsweeps));
@@ -693,10 +727,10 @@ int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off, const uin
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps_fused: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps_fused: null pair list");
   if (!d_img_off || !d_img) return fail(BK_ERR_ARG, "bk_sweeps_fused: null ghost-image table");
-  if (d->stencil == BK_CUBE125 || d->plan)
+  if (d->stencil == BK_CUBE125 || d->plan || d->layout != BK_LAYOUT_8x8x8)
     return fail(BK_ERR_UNSUPPORTED,
                 "bk_sweeps_fused: the fused refresh runs in the per-brick star/cond kernel "
-                "(no plan, no cube); use bk_sweeps");
+                "(8x8x8 bricks, no plan, no cube); use bk_sweeps");
   return sweeps_impl(d, d_pairs, npairs, nsweeps, 1, d->d_list, d->nlist, d->plan, d_img_off,
                      d_img, use_graph, S(s));
 }
@@ -709,11 +743,14 @@ int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps_deep: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps_deep: null pair list");
   const int R = bk_stencil_radius(d->stencil);
-  if (ghost_iters < 1 || R * ghost_iters > 8)
+  int ext[3];
+  layout_ext(valid_layout(d->layout) ? d->layout : BK_LAYOUT_8x8x8, ext);
+  const int depth = std::min(ext[0], std::min(ext[1], ext[2]));  // the ghost zone's thinnest side
+  if (ghost_iters < 1 || R * ghost_iters > depth)
     return fail(BK_ERR_ARG,
-                "bk_sweeps_deep: ghost_iters must be in [1, 8 / radius] (%d sweeps of radius %d "
-                "exceed the 8-cell ghost zone)",
-                ghost_iters, R);
+                "bk_sweeps_deep: ghost_iters must be in [1, %d / radius] (%d sweeps of radius %d "
+                "exceed the %d-cell ghost zone)",
+                depth, ghost_iters, R, depth);
   bk_sweep_desc all = *d;
   all.d_list = d_all;
   all.nlist = nall;
@@ -734,16 +771,22 @@ int bk_graph_cache_clear(void) {
 // ---------------------------------------------------------------------------
 // seeded fill and FP64 reductions
 // ---------------------------------------------------------------------------
-int bk_fill_uniform_grid(int dtype, void *d_dat, size_t step, const long nb[3], int ghost,
-                         uint64_t seed, double lo, double hi, bk_stream s) {
+int bk_fill_uniform_layout(int layout, int dtype, void *d_dat, size_t step, const long nb[3],
+                           int ghost, uint64_t seed, double lo, double hi, bk_stream s) {
+  if (!valid_layout(layout)) return fail(BK_ERR_ARG, "bk_fill_uniform: unknown layout %d", layout);
   if (!valid_dtype(dtype)) return fail(BK_ERR_ARG, "bk_fill_uniform: unknown dtype %d", dtype);
   if (!d_dat || !nb) return fail(BK_ERR_ARG, "bk_fill_uniform: null pointer");
   for (int a = 0; a < 3; ++a)
     if (nb[a] <= 0) return fail(BK_ERR_ARG, "bk_fill_uniform: nb must be positive");
   if (ghost != 0 && ghost != 1) return fail(BK_ERR_ARG, "bk_fill_uniform: ghost must be 0 or 1");
   if (step < 512) return fail(BK_ERR_ARG, "bk_fill_uniform: step smaller than a brick");
-  BK_HIP(bk::launch_fill_uniform(dtype, d_dat, step, nb, ghost, seed, lo, hi, S(s)));
+  BK_HIP(bk::launch_fill_uniform(layout, dtype, d_dat, step, nb, ghost, seed, lo, hi, S(s)));
   return BK_OK;
+}
+
+int bk_fill_uniform_grid(int dtype, void *d_dat, size_t step, const long nb[3], int ghost,
+                         uint64_t seed, double lo, double hi, bk_stream s) {
+  return bk_fill_uniform_layout(BK_LAYOUT_8x8x8, dtype, d_dat, step, nb, ghost, seed, lo, hi, s);
 }
 
 int bk_fill_uniform(int dtype, void *d_dat, size_t step, const long nb[3], uint64_t seed,
@@ -758,12 +801,13 @@ int bk_field_stats(int dtype, const void *d_dat, size_t step, const uint32_t *d_
   if (nlist && (!d_dat || !d_list)) return fail(BK_ERR_ARG, "bk_field_stats: null pointer");
   if (brick_elems == 0 || brick_elems > step)
     return fail(BK_ERR_ARG, "bk_field_stats: brick_elems must be in [1, step]");
-  auto &sc = scratch();
+  auto &sc = reduce_scratch();
   std::lock_guard<std::mutex> lock(sc.mu);
   double *buf = nullptr;
   int rc = get_scratch(&buf);
   if (rc != BK_OK) return rc;
-  BK_HIP(bk::launch_field_stats(dtype, d_dat, step, d_list, nlist, brick_elems, buf, S(s)));
+  BK_HIP(bk::launch_field_stats(dtype, d_dat, step, d_list, nlist, brick_elems, buf, nullptr,
+                                S(s)));
   BK_HIP(hipMemcpyAsync(h_out, buf, 3 * sizeof(double), hipMemcpyDeviceToHost, S(s)));
   BK_HIP(hipStreamSynchronize(S(s)));
   return BK_OK;
@@ -778,7 +822,7 @@ int bk_field_compare(int dtype_a, const void *d_a, size_t step_a, int dtype_r, c
   if (nlist && (!d_a || !d_r || !d_list)) return fail(BK_ERR_ARG, "bk_field_compare: null pointer");
   if (brick_elems == 0 || brick_elems > step_a || brick_elems > step_r)
     return fail(BK_ERR_ARG, "bk_field_compare: brick_elems must be in [1, step]");
-  auto &sc = scratch();
+  auto &sc = reduce_scratch();
   std::lock_guard<std::mutex> lock(sc.mu);
   double *buf = nullptr;
   int rc = get_scratch(&buf);

@@ -289,15 +289,18 @@ __host__ __device__ constexpr bool lds_half_tiles() {
   return sizeof(typename acc_of<T>::type) == 8;
 }
 
-template <class T, int R, bool COND>
-__global__ __launch_bounds__(256) void star_sweep_lds(const StarArgs<T, R, COND> a) {
+// WPB waves (bricks) per workgroup: 4 by default; the tuner also times 1,
+// 2 and 8 (the workgroup count per CU and the barrier group change, the
+// per-wave work does not).
+template <class T, int R, bool COND, int WPB = 4>
+__global__ __launch_bounds__(64 * WPB) void star_sweep_lds(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
   constexpr bool kHalf = lds_half_tiles<T, R>();
-  __shared__ A tile[4][(kHalf ? 4 : 8) * Tile<R, sizeof(A)>::PL];
+  __shared__ A tile[WPB][(kHalf ? 4 : 8) * Tile<R, sizeof(A)>::PL];
 
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * 4u + (uint32_t)w;
+  const uint32_t slot = xcd_remap(blockIdx.x, gridDim.x) * (uint32_t)WPB + (uint32_t)w;
   const bool active = slot < a.nlist;
 
   StarBrick<T, R, COND> sb;
@@ -511,6 +514,135 @@ __global__ __launch_bounds__(64) void star_sweep_columns_batched(const BatchTabl
 }
 
 
+// Quad-column schedule: one workgroup of four waves per quad-plan unit, a
+// 2x2 block of k-columns (members m0 = anchor, m1 = i+1, m2 = j+1,
+// m3 = i+1,j+1) marched level by level.  Each wave owns one member column
+// and keeps its planes in registers as the k-march does (the next level's
+// planes prefetched during compute, k-faces never read twice); the four
+// members' planes share one LDS tile of the whole 16x16 block plus its
+// R-deep outer faces, so the faces between the members come from LDS and
+// each wave fetches only its two outer faces: half the in-plane face bytes
+// of the per-column schedules.  Same per-point arithmetic.
+template <int R, int ES>
+struct QuadTile {
+  static constexpr int P = 16 + 2 * R;  // rows and columns of the block tile
+  static constexpr int RS = P;          // row stride
+  static constexpr int PL = P * RS;     // plane stride
+  __device__ static constexpr int at(int k, int y, int x) { return k * PL + y * RS + x; }
+};
+
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, COND> a) {
+  using A = typename acc_of<T>::type;
+  using QT = QuadTile<R, sizeof(A)>;
+  __shared__ A tile[8 * QT::PL];
+
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;  // uniform over the workgroup
+  const int mm = (int)uniform(threadIdx.x >> 6);  // member: i-offset mm & 1, j-offset mm >> 1
+  const int lane = threadIdx.x & 63;
+  const int j = lane >> 3, i = lane & 7;
+  const int mi = mm & 1, mj = mm >> 1;
+  const int L = (int)a.len[u];
+  const size_t st = a.in_step;
+  const uint32_t *adj = a.adj;
+  A c[6 * R + 1];
+#pragma unroll
+  for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
+
+  // this wave's level-0 brick
+  uint32_t b = a.anchor[u];
+  if (mi) b = adj[(size_t)b * 27 + 14];
+  if (mj) b = adj[(size_t)b * 27 + 16];
+  b = uniform(b);
+  // this member's outer neighbours: i-1 or i+1, j-1 or j+1
+  const int si = mi ? 14 : 12, sj = mj ? 16 : 10;
+  // face lanes: column c, depth rr, plane group kq (as Faces)
+  const int fc = lane & 7, frr = (lane >> 3) % R, fkq = (lane >> 3) / R;
+  // tile coordinates of this member's cells and of its outer faces
+  const int ty = R + 8 * mj + j, tx = R + 8 * mi + i;
+  const int fix = mi ? R + 16 + frr : frr;  // the i-face's tile column
+  const int fjy = mj ? R + 16 + frr : frr;  // the j-face's tile row
+
+  A cur[8], nxt[8], km[R];
+  {
+    const T *p = a.in + (size_t)b * st;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cur[k] = widen(p[k * 64 + lane]);
+    const T *q = a.in + (size_t)uniform(adj[(size_t)b * 27 + 4]) * st;
+#pragma unroll
+    for (int r = 0; r < R; ++r) km[r] = widen(q[(8 - R + r) * 64 + lane]);
+  }
+  for (int l = 0; l < L; ++l) {
+    const uint32_t *ab = adj + (size_t)b * 27;
+    const uint32_t up = uniform(ab[22]);
+    const bool more = l + 1 < L;
+    // the two outer faces first (waited for before the tile is filled),
+    // then the prefetch of the next level, which stays in flight through
+    // the compute below
+    A fi[R], fj[R];
+    {
+      const T *pi = a.in + (size_t)uniform(ab[si]) * st;
+      const T *pj = a.in + (size_t)uniform(ab[sj]) * st;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int k = q * (8 / R) + fkq;
+        fi[q] = widen(pi[k * 64 + fc * 8 + (mi ? frr : 8 - R + frr)]);
+        fj[q] = widen(pj[k * 64 + (mj ? frr : 8 - R + frr) * 8 + fc]);
+      }
+    }
+    const T *pu = a.in + (size_t)up * st;
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) nxt[k] = widen(pu[k * 64 + lane]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) nxt[r] = widen(pu[r * 64 + lane]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) tile[QT::at(k, ty, tx)] = cur[k];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = q * (8 / R) + fkq;
+      tile[QT::at(k, R + 8 * mj + fc, fix)] = fi[q];
+      tile[QT::at(k, fjy, R + 8 * mi + fc)] = fj[q];
+    }
+    __syncthreads();
+    if ((a.mask[(size_t)u * a.kcol + l] >> mm) & 1u) {
+      T *po = a.out + (size_t)b * a.out_step;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        auto X = [&](int kk) -> A {
+          return kk < 0 ? km[kk + R] : (kk >= 8 ? nxt[kk - 8] : cur[kk]);
+        };
+        A acc = c[0] * tapv<A, COND>(cur[k]);
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+          acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[QT::at(k, ty, tx + r)]), acc);
+          acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[QT::at(k, ty, tx - r)]), acc);
+          acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[QT::at(k, ty + r, tx)]), acc);
+          acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[QT::at(k, ty - r, tx)]), acc);
+          acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
+          acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
+        }
+        if constexpr (COND) {
+          const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (cur[k] > a.theta);
+          acc = sel ? acc : -acc;
+        }
+        store_nt(&po[k * 64 + lane], acc);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) km[r] = cur[8 - R + r];
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
+    }
+    b = up;
+    __syncthreads();  // every wave's reads of this level's tile are done
+  }
+}
+
 // K-stack variant: one workgroup of four waves per column unit of up to
 // four bricks stacked along k (plan kcol <= 4).  The four waves load their
 // bricks at the same time (the memory parallelism of the per-brick kernel)
@@ -619,6 +751,44 @@ hipError_t launch_cols_t(const SweepLaunch &L, const PlanView &P, hipStream_t s)
   else
     hipLaunchKernelGGL((star_sweep_columns<T, R, COND>), dim3(P.nunits), dim3(64), 0, s, a);
   return hipGetLastError();
+}
+
+template <class T, int R, bool COND>
+hipError_t launch_quads_t(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  using A = typename acc_of<T>::type;
+  StarArgs<T, R, COND> a;
+  a.adj = L.adj;
+  a.list = nullptr;
+  a.in = static_cast<const T *>(L.in);
+  a.out = static_cast<T *>(L.out);
+  a.in_step = L.in_step;
+  a.out_step = L.out_step;
+  a.nlist = 0;
+  a.cond_mode = L.cond_mode;
+  a.anchor = P.anchor;
+  a.len = P.len;
+  a.mask = P.mask;
+  a.flat = P.flat;
+  a.start = P.start;
+  a.nunits = P.nunits;
+  a.kcol = P.kcol;
+  a.img_off = a.img = nullptr;
+  a.theta = (A)L.theta;
+  for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
+  if (P.nunits == 0) return hipSuccess;
+  hipLaunchKernelGGL((star_sweep_quads<T, R, COND>), dim3(P.nunits), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_quads_dt(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  switch (L.stencil) {
+    case BK_STAR7: return launch_quads_t<T, 1, false>(L, P, s);
+    case BK_STAR13: return launch_quads_t<T, 2, false>(L, P, s);
+    case BK_STAR25: return launch_quads_t<T, 4, false>(L, P, s);
+    case BK_COND7: return launch_quads_t<T, 1, true>(L, P, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <class T>
@@ -789,8 +959,25 @@ hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   a.nunits = a.kcol = 0;
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
-  const uint32_t nwg = (L.nlist + 3u) / 4u;
-  hipLaunchKernelGGL((star_sweep_lds<T, R, COND>), dim3(nwg), dim3(256), 0, s, a);
+  switch (L.wg_bricks) {
+    case 1:
+      hipLaunchKernelGGL((star_sweep_lds<T, R, COND, 1>), dim3(L.nlist), dim3(64), 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((star_sweep_lds<T, R, COND, 2>), dim3((L.nlist + 1u) / 2u), dim3(128),
+                         0, s, a);
+      break;
+    case 8:
+      hipLaunchKernelGGL((star_sweep_lds<T, R, COND, 8>), dim3((L.nlist + 7u) / 8u), dim3(512),
+                         0, s, a);
+      break;
+    case 0:
+    case 4:
+      hipLaunchKernelGGL((star_sweep_lds<T, R, COND>), dim3((L.nlist + 3u) / 4u), dim3(256), 0,
+                         s, a);
+      break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -832,6 +1019,15 @@ hipError_t launch_star_columns_batched(int stencil, int dtype, const BatchTable 
     case BK_F64: return launch_cols_batched_dt<double>(stencil, tab, tick, s);
     case BK_F32: return launch_cols_batched_dt<float>(stencil, tab, tick, s);
     case BK_BF16: return launch_cols_batched_dt<bf16>(stencil, tab, tick, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_star_quads(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
+  switch (L.dtype) {
+    case BK_F64: return launch_quads_dt<double>(L, P, s);
+    case BK_F32: return launch_quads_dt<float>(L, P, s);
+    case BK_BF16: return launch_quads_dt<bf16>(L, P, s);
     default: return hipErrorInvalidValue;
   }
 }

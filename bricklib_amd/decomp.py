@@ -134,6 +134,9 @@ class DecompGrid(PeriodicGrid):
         self.sweep_list = self.interior
         self.d_sweep_list = self.d_interior
         self._plans = {}
+        self._tuned = {}
+        self.brick_layout = 0  # 8x8x8 bricks in the 8x8 fold
+        self.extents = (8, 8, 8)
 
     @property
     def ghost_images(self):

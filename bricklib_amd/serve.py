@@ -1,5 +1,8 @@
 """Serving loop: continuous batching of stencil solves on one GPU.
 
+The loop itself is C++ (bk_server_* of libbricks.so); this module is its
+Python handle plus the workload generator of the benchmark.
+
 Requests (grid size, stencil, dtype, sweep count, seed) arrive over time.
 The loop admits every arrived request whose fields fit the HBM budget
 (fields stay resident until the request completes), runs all admitted
@@ -20,14 +23,15 @@ on one lane and the stars on the other gained nothing).
 """
 from __future__ import annotations
 
-import time
+import ctypes as C
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import BK_CUBE125, BK_F32, BK_F64
-from .batch import Arena, Batch, Request, ShapeCache
-from .bricks import Event, Stream
+from . import _lib
+from ._lib import BK_CUBE125, BK_F32, BK_F64, check, load
+from .batch import Request
+from .bricks import _vp
 from .stencils import NAMES, Stencil, u01
 
 
@@ -56,167 +60,139 @@ def workload(n: int, rate: float, seed: int = 2024, edges=(32, 256), max_steps: 
 
 @dataclass
 class ServeStats:
+    """What a serve() call completed and the server's statistics for it
+    (bk_server_stats)."""
+
     completed: list = field(default_factory=list)
     elapsed: float = 0.0
-    peak_bytes: int = 0
-    reserved_bytes: int = 0  # the arena's slabs at the end
+    peak_bytes: float = 0.0
+    reserved_bytes: float = 0.0  # the arena's slabs at the end
     chunks: int = 0
+    native: dict = field(default_factory=dict)
 
     def latencies(self) -> np.ndarray:
         return np.array([r.finished - r.arrival for r in self.completed])
 
     def summary(self) -> dict:
-        lat = self.latencies()
-        work = sum(r.cells * r.steps for r in self.completed)
+        n = self.native
         return {"requests": len(self.completed),
-                "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
-                "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
-                "mean_latency_s": float(lat.mean()) if lat.size else None,
+                "p50_latency_s": n.get("p50_s") if self.completed else None,
+                "p99_latency_s": n.get("p99_s") if self.completed else None,
+                "mean_latency_s": n.get("mean_s") if self.completed else None,
                 "elapsed_s": self.elapsed,
-                "gstencil_s": work / self.elapsed / 1e9 if self.elapsed > 0 else 0.0,
-                "stencil_updates": work, "peak_hbm_bytes": self.peak_bytes,
-                "arena_reserved_bytes": self.reserved_bytes,
-                "chunks": self.chunks}
+                "gstencil_s": n.get("gstencil_s", 0.0),
+                "stencil_updates": int(n.get("stencil_updates", 0)),
+                "peak_hbm_bytes": int(self.peak_bytes),
+                "arena_reserved_bytes": int(self.reserved_bytes),
+                "chunks": self.chunks, "tuned_classes": n.get("tuned_classes", 0)}
 
 
 def lane_of(kind: int, dtype: int, lanes: int) -> int:
     """The lane of a request with `lanes` lanes: 0 for FP64 cubes and the
     stars (and cond) of the other dtypes, 1 for the other cubes and the
-    FP64 stars; 0 with one lane."""
+    FP64 stars; 0 with one lane (the rule of the C++ loop)."""
     if lanes == 1:
         return 0
     return 0 if (kind == BK_CUBE125) == (dtype == BK_F64) else 1
 
 
 class Server:
-    """Continuous-batching loop over `lanes` batches (one stream each)."""
+    """Python handle of the C++ serving loop (bk_server_*): continuous
+    batching over `lanes` batches (one stream each), requests admitted
+    against an HBM budget, chunks of `chunk` ticks, the next chunk queued
+    before the current one is waited for (`pipeline`), and with `tune` each
+    request's schedule chosen by the autotuner per size class."""
 
     def __init__(self, hbm_budget: float = 200e9, chunk: int = 16, max_active: int = 2048,
                  use_graph: bool = True, ghost: int = 0, slab_bytes: int = 1 << 30,
-                 lanes: int = 2, pipeline: bool = True):
+                 lanes: int = 2, pipeline: bool = True, tune: bool = False):
         if lanes not in (1, 2):
             raise ValueError("lanes must be 1 or 2")
-        self.streams = [Stream() for _ in range(lanes)]
-        self.stream = self.streams[0]
-        # request fields come from an arena; a retired request's memory is
-        # reused at once, without a driver call: blocks are released only
-        # after every lane's stream has finished the chunk
-        self.arena = Arena(slab_bytes)
-        self.batches = [Batch(max_active, stream=s) for s in self.streams]
-        self.batch = self.batches[0]
-        self.shapes = ShapeCache()
+        cfg = _lib.bk_serve_config()
+        cfg.hbm_budget = float(hbm_budget)
+        cfg.chunk = int(chunk)
+        cfg.max_active = int(max_active)
+        cfg.lanes = int(lanes)
+        cfg.use_graph = int(bool(use_graph))
+        cfg.ghost = int(ghost)
+        cfg.pipeline = int(bool(pipeline))
+        cfg.tune = int(bool(tune))
+        cfg.slab_bytes = int(slab_bytes)
+        h = C.c_void_p()
+        check(load().bk_server_create(C.byref(cfg), C.byref(h)))
+        self.handle = h.value
         self.budget = float(hbm_budget)
-        self.chunk = int(chunk)
-        self.max_active = int(max_active)
-        self.use_graph = use_graph
-        # pipeline: queue the next chunk before waiting for the current one
-        # (the host's retire/admit/table work then overlaps the GPU; requests
-        # that arrive meanwhile join one chunk later).  At 200 req/s: p50
-        # 8.75 -> 8.6 ms, p99 47.9 -> 46 ms, saturating 0.362 -> 0.359 s.
-        self.pipeline = bool(pipeline)
-        # field layout of admitted requests: 0 = wrapped adjacency (no ghost
-        # bricks, no refresh), 1 = one-brick ghost zone refreshed every tick
         self.ghost = int(ghost)
+        self.lanes = int(lanes)
+        self.chunk = int(chunk)
 
     def lane(self, r: Request) -> int:
-        return lane_of(r.stencil.kind, r.dtype, len(self.batches))
+        return lane_of(r.stencil.kind, r.dtype, self.lanes)
 
-    def admissible(self, r: Request, used: int, nactive: int) -> bool:
-        return nactive < self.max_active and used + r.hbm_bytes <= self.budget
+    def _spec(self, r: Request, keep: bool) -> "_lib.bk_serve_req":
+        q = _lib.bk_serve_req()
+        for a in range(3):
+            q.nb[a] = int(r.nb[a])
+        q.stencil = int(r.stencil.kind)
+        q.dtype = int(r.dtype)
+        q.steps = int(r.steps)
+        q.seed = int(r.seed)
+        r._coeff = np.ascontiguousarray(r.stencil.coeff, np.float64)
+        q.h_coeff = r._coeff.ctypes.data_as(_lib.dblp)
+        q.theta = float(r.stencil.theta)
+        q.cond_mode = int(r.stencil.cond_mode)
+        q.arrival = float(r.arrival)
+        q.keep = int(bool(keep))
+        return q
 
     def prewarm(self, requests: list) -> None:
-        """Build the shared per-shape tables of these requests' grid shapes
-        ahead of time (a shape catalogue known to the server)."""
+        """Build the shared per-shape tables (and, with tuning, time the
+        size classes) of these requests ahead of serving."""
         for r in requests:
-            self.shapes.get(r.nb, self.ghost)
-        self.stream.sync()
+            r.ghost = self.ghost
+        specs = (_lib.bk_serve_req * len(requests))(*[self._spec(r, False) for r in requests])
+        check(load().bk_server_prewarm(_vp(self.handle), specs, len(requests)))
 
-    def serve(self, requests: list, realtime: bool = True, keep_results=False,
-              on_complete=None) -> ServeStats:
-        """Serve `requests` (sorted by arrival).  realtime: arrivals follow
-        the wall clock from the start; otherwise all are present at t=0
-        (a saturating, closed-loop run).  keep_results: True, or a set of
-        request ids whose fields stay allocated after completion."""
-        pending = sorted(requests, key=lambda r: r.arrival)
-        if not realtime:
-            for r in pending:
-                r.arrival = 0.0
-        for r in pending:
+    def serve(self, requests: list, realtime: bool = True, keep_results=False) -> ServeStats:
+        """Serve `requests` to completion (bk_server_run).  realtime: each is
+        admitted once the clock from the start passes its arrival; otherwise
+        all are present at t=0 (a saturating, closed-loop run).
+        keep_results: True, or a set of request ids whose result fields stay
+        allocated (Request.interior_host / release)."""
+        for r in requests:
             r.ghost = self.ghost
             if r.hbm_bytes > self.budget:
                 raise ValueError(f"request {r.rid} needs {r.hbm_bytes} B, above the budget")
-        active: list = []
-        used = 0
-        st = ServeStats()
-        t0 = time.perf_counter()
-        qi = 0
-        inflight = None  # pipelined: (events, finishing requests) of the queued chunk
+        lib = load()
+        check(lib.bk_server_reset_stats(_vp(self.handle)))
+        for r in sorted(requests, key=lambda r: r.arrival):
+            keep = keep_results if isinstance(keep_results, bool) else r.rid in keep_results
+            q = self._spec(r, keep)
+            sid = C.c_uint32()
+            check(lib.bk_server_submit(_vp(self.handle), C.byref(q), C.byref(sid)))
+            r._server, r._sid = self, sid.value
+        check(lib.bk_server_run(_vp(self.handle), int(bool(realtime))))
+        info = _lib.bk_serve_info()
+        for r in requests:
+            check(lib.bk_server_info(_vp(self.handle), r._sid, C.byref(info)))
+            r.arrival, r.admitted, r.finished = info.arrival, info.admitted, info.finished
+            r.done = info.sweeps_done
+            r.schedule = (info.plan_shape, info.kcol)
+        c = _lib.bk_serve_stats()
+        check(lib.bk_server_stats(_vp(self.handle), C.byref(c)))
+        native = {f: getattr(c, f) for f, _ in _lib.bk_serve_stats._fields_}
+        return ServeStats(completed=sorted(requests, key=lambda r: r.finished),
+                          elapsed=c.elapsed_s, peak_bytes=c.peak_hbm_bytes,
+                          reserved_bytes=c.arena_reserved_bytes, chunks=int(c.chunks),
+                          native=native)
 
-        def retire(done: list) -> None:
-            nonlocal used
-            now = time.perf_counter() - t0
-            for r in done:
-                r.finished = now
-                if on_complete is not None:
-                    on_complete(r)
-                keep = keep_results if isinstance(keep_results, bool) else (
-                    r.rid in keep_results)
-                if not keep:
-                    r.release()
-                used -= r.hbm_bytes
-                st.completed.append(r)
-
-        while qi < len(pending) or active or inflight is not None:
-            now = time.perf_counter() - t0
-            # admit arrived requests that fit
-            while qi < len(pending) and pending[qi].arrival <= now:
-                r = pending[qi]
-                if not self.admissible(r, used, len(active)):
-                    break
-                # filled on its lane's stream, which runs all its ticks
-                r.allocate(self.shapes, self.streams[self.lane(r)], self.arena)
-                r.admitted = now
-                used += r.hbm_bytes
-                active.append(r)
-                qi += 1
-            st.peak_bytes = max(st.peak_bytes, used)
-            if not active and inflight is None:
-                if qi < len(pending):
-                    time.sleep(max(0.0, pending[qi].arrival - (time.perf_counter() - t0)))
-                continue
-            queued = None
-            if active:
-                ticks = min(self.chunk, max(r.remaining for r in active))
-                lanes = [[] for _ in self.batches]
-                for r in active:
-                    lanes[self.lane(r)].append(r)
-                for b, reqs in zip(self.batches, lanes):
-                    if reqs:
-                        b.run(reqs, ticks, self.use_graph)
-                st.chunks += 1
-                done = [r for r in active if r.remaining == 0]
-                active = [r for r in active if r.remaining > 0]
-                if self.pipeline:
-                    events = []
-                    for stream in self.streams:
-                        e = Event()
-                        e.record(stream)
-                        events.append(e)
-                    queued = (events, done)
-                else:
-                    for stream in self.streams:
-                        stream.sync()
-                    retire(done)
-            if inflight is not None:
-                # the previous chunk, queued before this one: its requests
-                # are complete once its events are
-                for e in inflight[0]:
-                    e.sync()
-                retire(inflight[1])
-            inflight = queued
-        st.elapsed = time.perf_counter() - t0
-        st.reserved_bytes = self.arena.stats()["reserved"]
-        return st
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().bk_server_destroy(_vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass
 
 
 def stencil_name(kind: int) -> str:

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define BK_API_VERSION 1
+#define BK_API_VERSION 2
 
 enum bk_status {
   BK_OK = 0,
@@ -71,6 +71,18 @@ enum bk_stencil {
 
 enum bk_cond_mode { BK_COND_REFERENCE = 0, BK_COND_THRESHOLD = 1 };
 
+/* Brick shape and vector fold of a field: Brick<Dim<BK,BJ,BI>,Dim<FJ,FI>>
+ * (brick.h:353-395).  Element (k, j, i) of a brick sits at
+ *   vec*64 + (j % FJ)*FI + i % FI,  vec = (k*(BJ/FJ) + j/FJ)*(BI/FI) + i/FI
+ * (the accessor's fold decomposition, brick.h:235-246, 276-284).  Every
+ * shape holds 512 elements; grids count bricks per axis as usual, so a
+ * 4x8x16 grid of n^3 cells has n/16 x n/8 x n/4 bricks (i, j, k).      */
+enum bk_layout {
+  BK_LAYOUT_8x8x8 = 0,      /* Dim<8,8,8>,  Dim<8,8>:  k*64 + j*8 + i          */
+  BK_LAYOUT_4x8x16 = 1,     /* Dim<4,8,16>, Dim<4,16>: k*128 + j*16 + i        */
+  BK_LAYOUT_4x8x16_F88 = 2  /* Dim<4,8,16>, Dim<8,8>:  k*128 + (i/8)*64 + j*8 + i%8 */
+};
+
 typedef void *bk_stream; /* hipStream_t */
 typedef void *bk_event;  /* hipEvent_t */
 typedef struct bk_plan_s *bk_plan; /* execution plan (bk_plan_create) */
@@ -84,6 +96,9 @@ int bk_version(void);
 const char *bk_last_error(void);
 /* Number of coefficients a stencil takes (0 for an unknown kind). */
 int bk_stencil_ncoeff(int stencil);
+/* Brick extents of a layout: ext[0] = BK (k), ext[1] = BJ, ext[2] = BI;
+ * BK_ERR_ARG for an unknown layout. */
+int bk_layout_extents(int layout, int ext[3]);
 /* Radius of a stencil (max |offset| along an axis), 0 for an unknown kind. */
 int bk_stencil_radius(int stencil);
 int bk_dtype_size(int dtype);
@@ -233,6 +248,17 @@ int bk_decomp_layout_skin(const long nb[3], int ghost_bricks, size_t brick_bytes
  * h_list may be NULL to query *n. */
 int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n);
 
+/* The interior bricks of a grid of nb bricks per axis with `ghost` (0 or 1)
+ * ghost layers, numbered as bk_init_grid over nb + 2*ghost, in the order
+ * the sweeps visit them best: the (i, j) plane cut into tile_i x tile_j
+ * brick tiles, each swept through all k before the next (k-neighbours are
+ * then tile_i*tile_j list positions apart instead of a whole brick plane;
+ * 0 means the whole axis, so 0, 0 is grid order with k slowest).  The
+ * reference loops over grid[k][j][i] (stencils/3axis.cpp:51-66); any order
+ * gives the same results.  h_list may be NULL to query *n. */
+int bk_tiled_list(const long nb[3], int ghost, uint32_t tile_i, uint32_t tile_j,
+                  uint32_t *h_list, uint32_t *n);
+
 /* ---- execution plans --------------------------------------------------- */
 /* A plan is a schedule for sweeping the bricks of a list: it groups them,
  * through the host copy of the adjacency, into units one wave marches
@@ -254,9 +280,11 @@ int bk_interior_list(const long nb[3], uint32_t *h_list, uint32_t *n);
  * Kernel choice follows the plan: star stencils run a column plan with
  * kcol <= 4 as k-stacks (one workgroup loads the stacked bricks together
  * and shares their k-faces in LDS) and with kcol > 4 as k-marches (one wave
- * walks the column, prefetching the next brick); the cube streams a quad
- * plan plane by plane; any other pairing, and a sweep without a plan, runs
- * one brick per wave.  Results do not depend on the choice. */
+ * walks the column, prefetching the next brick); star stencils run a quad
+ * plan as 2x2 blocks of k-marching columns, one workgroup per block whose
+ * four waves share an LDS tile of the block (only its outer faces are
+ * fetched); the cube streams a quad plan plane by plane; a sweep without a
+ * plan runs one brick per wave.  Results do not depend on the choice. */
 int bk_plan_create(const uint32_t *h_adj, uint32_t nbricks,
                    const uint32_t *h_list, uint32_t nlist, int shape,
                    uint32_t kcol, bk_plan *plan);
@@ -293,6 +321,10 @@ typedef struct bk_sweep_desc {
   int cond_mode;          /* enum bk_cond_mode */
   bk_plan plan;           /* optional: when set, the sweep covers the plan's
                              bricks and d_list / nlist are ignored */
+  int layout;             /* enum bk_layout of both fields (0: 8x8x8) */
+  int wg_bricks;          /* bricks (waves) per workgroup of the per-brick
+                             kernels: 0 (default, 4), 1, 2, 4 or 8 (8: the
+                             8x8x8 stars only) */
 } bk_sweep_desc;
 
 /* One stencil sweep: for every brick b in d_list (or in the plan) and
@@ -300,7 +332,9 @@ typedef struct bk_sweep_desc {
  * out[b] = stencil(in) read through the adjacency.  This is the loop of
  * stencils/3axis.cpp:51-66 (7pt), :123-140 (cond), and the brick(...)
  * kernel of weak/main.hip.cpp:32-40 (13/25/125-point via
- * stencils/mpi*.py), for 8x8x8 bricks in the 8x8 vector fold.
+ * stencils/mpi*.py), for 8x8x8 bricks in the 8x8 vector fold or, with
+ * d->layout, 4x8x16 bricks in the 4x16 or 8x8 fold (no plan: one brick per
+ * wave, d->wg_bricks bricks per workgroup).
  * Accumulation order per point (fixed, so results are reproducible):
  * acc = c_t0 * x_t0, then acc = fma(c_t, x_t, acc) over the remaining taps,
  * in the coefficient order above for stars and (dk, dj, di) lexicographic
@@ -330,7 +364,8 @@ int bk_sweeps(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs,
  * last sweep of each group and of the sequence covers d->d_list (or
  * d->plan).  Stale ghost values move inwards by the stencil radius per
  * sweep, so the interior is exact -- bit for bit the bk_sweeps result --
- * when radius * ghost_iters <= 8 (the one-brick ghost zone); larger values
+ * when radius * ghost_iters <= 8 (the one-brick ghost zone; 4 for the
+ * 4x8x16 layouts, whose bricks are 4 deep along k); larger values
  * are rejected.  ghost_iters = 1 is bk_sweeps.  all_plan: a column plan
  * over d_all, or NULL (one brick per wave); bk_plan_create rejects quad
  * plans over the outer layer of a non-periodic grid. */
@@ -349,6 +384,52 @@ int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off,
                     int nsweeps, int use_graph, bk_stream s);
 /* Drop every cached graph (frees their device resources). */
 int bk_graph_cache_clear(void);
+
+/* ---- autotuner ---------------------------------------------------------- */
+/* One timed schedule: brick layout, plan (0: none, one brick per wave) and
+ * its length, bricks per workgroup of the per-brick kernel, and the best
+ * measured milliseconds per sweep (refresh included on a ghost layout). */
+typedef struct bk_tune_cand {
+  int layout;      /* enum bk_layout */
+  int plan_shape;  /* 0, BK_PLAN_COLUMN or BK_PLAN_QUAD */
+  uint32_t kcol;   /* plan length (0 without a plan) */
+  int wg_bricks;   /* bricks per workgroup without a plan (0 with one) */
+  float ms;
+} bk_tune_cand;
+
+/* flags of bk_tune_sweep: the layouts to try (bit l = enum bk_layout l;
+ * at least one), and a restriction to the schedules the batched kernels
+ * run (8x8x8 only: per brick with 4 bricks per workgroup or k-columns for
+ * the stars, 2x2 column quads for the cube). */
+#define BK_TUNE_LAYOUT_8x8x8 0x1u
+#define BK_TUNE_LAYOUT_4x8x16 0x2u
+#define BK_TUNE_LAYOUT_4x8x16_F88 0x4u
+#define BK_TUNE_ALL_LAYOUTS 0x7u
+#define BK_TUNE_BATCH_SCHEDULES 0x100u
+
+/* The autotuner (no counterpart in the reference, whose brick shape, fold
+ * and launch shape are fixed at compile time: brick.h:353-395,
+ * stencils/stencils.h:10-12, 3axis.hip.cpp:16-41).  On a periodic grid of
+ * cells[0] x cells[1] x cells[2] cells (i, j, k) with `ghost` (0: wrapped
+ * adjacency, 1: one ghost-brick layer refreshed before every sweep) it
+ * builds the grid in every selected layout the extents fit, runs every
+ * schedule of that layout -- per brick with 1, 2, 4 or 8 bricks per
+ * workgroup (8: 8x8x8 stars only; 4: the cube on 8x8x8), k-columns of
+ * 4/8/16 bricks and 2x2 column quads of 4/8 levels (8x8x8 stars), quads of
+ * 2/4/8 levels (8x8x8 cube) -- untimed for warm_s seconds, then times `steps` ping-pong sweeps
+ * per candidate (hipGraph replays) in `rounds` interleaved rounds, keeping
+ * each candidate's best.  h_out receives up to max_out candidates fastest
+ * first (h_out[0] is the choice); with h_out NULL *nout is the candidate
+ * count and nothing runs.  Allocates and frees its own grids; the result
+ * is also kept for bk_tune_lookup.  All schedules compute the same bits. */
+int bk_tune_sweep(int stencil, int dtype, const double *h_coeff, double theta,
+                  int cond_mode, const long cells[3], int ghost, int steps,
+                  int rounds, double warm_s, unsigned flags, bk_tune_cand *h_out,
+                  uint32_t max_out, uint32_t *nout, bk_stream s);
+/* The fastest candidate of an earlier bk_tune_sweep with the same stencil,
+ * dtype, cells, ghost and flags; BK_ERR_ARG when there is none. */
+int bk_tune_lookup(int stencil, int dtype, const long cells[3], int ghost,
+                   unsigned flags, bk_tune_cand *best);
 
 /* ---- batched sweeps over many grids ------------------------------------ */
 /* One request of a batch: an independent grid with its own fields,
@@ -398,6 +479,94 @@ int bk_batch_set(bk_batch batch, const bk_batch_req *reqs, uint32_t n, bk_stream
  * the kernels run, so later bk_batch_set calls need no new capture. */
 int bk_batch_run(bk_batch batch, uint32_t nticks, int use_graph, bk_stream s);
 
+/* ---- serving loop ------------------------------------------------------ */
+/* Continuous batching of stencil solves on one GPU (no counterpart in the
+ * reference; its closest precedent is the strong-scaling driver's one
+ * launch over many subdomains, strong/main.cu:85-99).  Requests of any grid
+ * size, stencil, dtype and sweep count are admitted as they arrive against
+ * an HBM budget (two fields each, from a field arena, resident until the
+ * request completes), swept together chunk by chunk through bk_batch_run on
+ * one or two lanes (streams), and retired as they complete, freeing their
+ * memory for the next arrivals. */
+typedef struct bk_server_s *bk_server;
+
+typedef struct bk_serve_config {
+  double hbm_budget;    /* bytes of fields resident at once */
+  uint32_t chunk;       /* ticks per batched run between admissions (16) */
+  uint32_t max_active;  /* requests resident at once, <= 2048 */
+  int lanes;            /* 1 or 2 batches on their own streams */
+  int use_graph;        /* replay each chunk's launches as a hipGraph */
+  int ghost;            /* 0: wrapped adjacency (no refresh), 1: ghost layer */
+  int pipeline;         /* queue the next chunk before waiting for the last */
+  int tune;             /* schedules from bk_tune_sweep per (size class,
+                           stencil, dtype); 0: fixed defaults */
+  size_t slab_bytes;    /* arena slab size (0: 1 GiB) */
+} bk_serve_config;
+
+typedef struct bk_serve_req {
+  long nb[3];            /* interior bricks per axis (i, j, k) */
+  int stencil;           /* enum bk_stencil */
+  int dtype;             /* enum bk_dtype */
+  uint32_t steps;        /* sweeps */
+  uint64_t seed;         /* input: bk_fill_uniform_grid(seed, -1, 1) */
+  const double *h_coeff; /* copied by bk_server_submit */
+  double theta;
+  int cond_mode;
+  double arrival;        /* seconds on the server clock (< 0: now) */
+  int keep;              /* keep the result field until bk_server_release */
+} bk_serve_req;
+
+typedef struct bk_serve_info {
+  int state;            /* 0 pending, 1 active, 2 done (fields kept), 3 done */
+  double arrival, admitted, finished; /* server clock, seconds */
+  uint32_t sweeps_done;
+  size_t field_bytes;
+  const void *d_result; /* state 2: the field holding the result */
+  int plan_shape;       /* the schedule it ran: 0 (per brick) or a plan shape */
+  uint32_t kcol;
+} bk_serve_info;
+
+typedef struct bk_serve_stats {
+  uint32_t requests;    /* completed since the last reset */
+  double p50_s, p99_s, mean_s; /* latency = finished - arrival */
+  double elapsed_s;     /* from the run's start to the last completion */
+  double stencil_updates; /* sum of interior cells x sweeps */
+  double gstencil_s;
+  double peak_hbm_bytes;
+  double arena_reserved_bytes;
+  uint64_t chunks;
+  uint32_t tuned_classes;
+} bk_serve_stats;
+
+int bk_server_create(const bk_serve_config *cfg, bk_server *out);
+/* Stops the thread (bk_server_stop), frees every field and table. */
+int bk_server_destroy(bk_server s);
+/* Queue a request (thread-safe); *id numbers requests from 0 in submission
+ * order.  A request whose fields exceed the budget is refused. */
+int bk_server_submit(bk_server s, const bk_serve_req *r, uint32_t *id);
+/* Build the shared tables (adjacency, brick list, ghost pairs, plans) of
+ * these requests' grid shapes, and with tuning on time their size classes,
+ * ahead of serving (the server's shape catalogue).  Nothing is queued. */
+int bk_server_prewarm(bk_server s, const bk_serve_req *r, uint32_t n);
+/* Serve every queued request to completion on the calling thread, the
+ * clock starting now: with realtime != 0 a request is admitted once the
+ * clock passes its arrival, otherwise all are present at once (a
+ * saturating run). */
+int bk_server_run(bk_server s, int realtime);
+/* Or serve on the server's own thread: start it, submit from any thread,
+ * wait for a request, stop (returns once everything queued is done). */
+int bk_server_start(bk_server s);
+int bk_server_stop(bk_server s);
+int bk_server_wait(bk_server s, uint32_t id, double timeout_s);
+int bk_server_info(bk_server s, uint32_t id, bk_serve_info *out);
+/* Copy a completed kept request's result field (all its bricks, numbered
+ * as bk_init_grid over nb + 2*ghost) to the host; bytes = field_bytes. */
+int bk_server_result(bk_server s, uint32_t id, void *h_out, size_t bytes);
+/* Return a kept request's fields to the arena. */
+int bk_server_release(bk_server s, uint32_t id);
+int bk_server_stats(bk_server s, bk_serve_stats *out);
+int bk_server_reset_stats(bk_server s);
+
 /* ---- seeded input and FP64 reductions ---------------------------------- */
 /* Fill the interior of a padded brick grid (nb interior bricks per axis,
  * one ghost layer) with the seeded field value(x,y,z) = lo + (hi-lo) * u,
@@ -413,6 +582,12 @@ int bk_fill_uniform(int dtype, void *d_dat, size_t step, const long nb[3],
 int bk_fill_uniform_grid(int dtype, void *d_dat, size_t step, const long nb[3],
                          int ghost, uint64_t seed, double lo, double hi,
                          bk_stream s);
+/* bk_fill_uniform_grid for any layout: nb interior bricks per axis of the
+ * layout's shape, cell (x, y, z) of the nb[0]*BI x nb[1]*BJ x nb[2]*BK
+ * interior gets the value bk_fill_uniform gives that cell. */
+int bk_fill_uniform_layout(int layout, int dtype, void *d_dat, size_t step,
+                           const long nb[3], int ghost, uint64_t seed, double lo,
+                           double hi, bk_stream s);
 /* FP64 reductions over the bricks in d_list (brick_elems each):
  *   out[0] = sum x, out[1] = sum x^2, out[2] = max |x|
  * written to the host array h_out after a stream sync. */

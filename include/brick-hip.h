@@ -40,6 +40,17 @@ inline void bkCheck(int rc, const char *what) {
   if (rc != BK_OK) throw std::runtime_error(std::string(what) + ": " + bk_last_error());
 }
 
+/// Throws unless libbricks.so speaks the C-ABI version these headers were
+/// written for (a descriptor struct laid out for another version would be
+/// misread).
+inline void bkCheckVersion() {
+  static const int v = bk_version();
+  if (v != BK_API_VERSION)
+    throw std::runtime_error("libbricks.so has C-ABI version " + std::to_string(v) +
+                             ", the headers expect " + std::to_string(BK_API_VERSION) +
+                             ": rebuild the program against the library's include/");
+}
+
 namespace bk_detail {
 
 /// Throws unless p is device memory (a kernel given a host pointer would
@@ -263,6 +274,7 @@ void check_layout() {
 template <typename BrickT>
 bk_sweep_desc make_desc(const BrickStencil &st, BrickT &bOut, BrickT &bIn,
                         const unsigned *d_list, unsigned nlist, bk_plan plan) {
+  bkCheckVersion();
   check_layout<BrickT>();
   if (!bIn.bStorage.on_device || !bOut.bStorage.on_device)
     throw std::invalid_argument("brick_sweep: fields must be device-resident (movBrickStorage)");

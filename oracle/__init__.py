@@ -53,6 +53,14 @@ _SIG = {
     "or_field_compare": (None, [C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_size_t, u32p,
                                 C.c_uint32, C.c_size_t, C.c_double, C.c_double, C.c_double,
                                 dblp]),
+    "or_layout_extent": (C.c_int, [C.c_int, C.c_int]),
+    "or_layout_offset": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "or_fill_uniform_layout": (None, [C.c_int, C.c_int, vp, C.c_size_t, longp, C.c_int,
+                                      C.c_uint64, C.c_double, C.c_double]),
+    "or_to_dense_layout": (None, [C.c_int, C.c_int, vp, C.c_size_t, longp, C.c_int, dblp]),
+    "or_brick_sweep_layout": (None, [C.c_int, C.c_int, C.c_int, C.c_int, u32p, u32p,
+                                     C.c_uint32, vp, C.c_size_t, vp, C.c_size_t, dblp,
+                                     C.c_double, C.c_int]),
     "or_num_threads": (C.c_int, []),
     "or_set_num_threads": (None, [C.c_int]),
     "or_wtime": (C.c_double, []),
@@ -219,6 +227,59 @@ class Grid(Layout):
                     b = (ti + 1) + S0 * ((tj + 1) + S1 * (tk + 1))
                     dat[b, offset:offset + 512] = arr[tk * 8:(tk + 1) * 8, tj * 8:(tj + 1) * 8,
                                                       ti * 8:(ti + 1) * 8].reshape(-1)
+
+
+LAYOUT_888, LAYOUT_4816, LAYOUT_4816_F88 = 0, 1, 2
+
+
+def layout_extents(layout):
+    """(k, j, i) extents of a layout's bricks."""
+    return tuple(lib().or_layout_extent(layout, a) for a in range(3))
+
+
+class LayoutGrid(Layout):
+    """A periodic grid of `nb` interior bricks per axis (contiguous-first) in
+    a brick layout (0: 8x8x8 fold 8x8, 1: 4x8x16 fold 4x16, 2: 4x8x16 fold
+    8x8), numbered by init_grid over (nb+2) bricks per axis with one ghost
+    layer, or over nb with the adjacency wrapped (ghost=0)."""
+
+    def __init__(self, nb, layout, ghost=1):
+        self.nb = tuple(int(x) for x in nb)
+        self.layout = int(layout)
+        self.ghost = int(ghost)
+        self.ext = layout_extents(self.layout)  # (BK, BJ, BI)
+        self.cells = (self.nb[0] * self.ext[2], self.nb[1] * self.ext[1],
+                      self.nb[2] * self.ext[0])  # contiguous-first
+        if ghost:
+            self.padded = tuple(x + 2 for x in self.nb)
+            grid, adj = init_grid(self.padded)
+            interior, pairs = interior_list(self.nb), ghost_pairs(self.nb)
+        else:
+            self.padded = self.nb
+            grid, adj = init_grid(self.nb, periodic=True)
+            interior, pairs = np.arange(grid.size, dtype=np.uint32), np.zeros(0, np.uint32)
+        self.grid = grid
+        super().__init__(adj, interior, pairs)
+
+    def fill_uniform(self, dat, seed, lo=-1.0, hi=1.0, dtype=F64):
+        lib().or_fill_uniform_layout(self.layout, dtype, P(dat), dat.shape[1], L3(self.nb),
+                                     self.ghost, seed, lo, hi)
+
+    def to_dense(self, dat, dtype=F64):
+        """the interior as a dense FP64 (n2, n1, n0) array, x fastest"""
+        out = np.empty((self.cells[2], self.cells[1], self.cells[0]), np.float64)
+        lib().or_to_dense_layout(self.layout, dtype, P(dat), dat.shape[1], L3(self.nb),
+                                 self.ghost, P(out, dblp))
+        return out
+
+    def sweep(self, stencil, coeff, inp, out, dtype=F64, mode=MODE_FMA, theta=0.0,
+              cond_mode=0, in_off=0, out_off=0, blist=None):
+        coeff = np.ascontiguousarray(coeff, np.float64)
+        blist = self.interior if blist is None else np.ascontiguousarray(blist, np.uint32)
+        lib().or_brick_sweep_layout(self.layout, stencil, dtype, mode, P(self.adj, u32p),
+                                    P(blist, u32p), blist.size, P(inp.reshape(-1)[in_off:]),
+                                    inp.shape[1], P(out.reshape(-1)[out_off:]), out.shape[1],
+                                    P(coeff, dblp), theta, cond_mode)
 
 
 def field_compare(a, r, interior, dta=F64, dtr=F64, floor=1.0, atol=0.0, rtol=0.0):

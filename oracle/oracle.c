@@ -473,6 +473,134 @@ void or_brick_sweep(int st, int dt, int mode, const uint32_t *adj,
   }
 }
 
+/* ---------------------------------------------------------------------- */
+/* other brick shapes and vector folds: Brick<Dim<BK,BJ,BI>,Dim<FJ,FI>>    */
+/* ---------------------------------------------------------------------- */
+/* Layouts (the library's bk_layout):
+ *   0  Dim<8,8,8>,  Dim<8,8>   element k*64 + j*8 + i
+ *   1  Dim<4,8,16>, Dim<4,16>  element k*128 + j*16 + i (row-major)
+ *   2  Dim<4,8,16>, Dim<8,8>   element k*128 + (i/8)*64 + j*8 + i%8
+ * The element offsets follow the reference accessor's fold decomposition
+ * (brick.h:276-284 with the vector index of brick.h:235-246): the vector
+ * index runs over the brick divided by the fold, outermost first, and the
+ * lane over the fold itself.  For fold (FJ, FI) of a (BK, BJ, BI) brick:
+ * vector ((k*(BJ/FJ) + j/FJ)*(BI/FI) + i/FI), lane (j%FJ)*FI + i%FI. */
+#define OR_LAYOUTS 3
+static const int lay_ext[OR_LAYOUTS][3] = {{8, 8, 8}, {4, 8, 16}, {4, 8, 16}}; /* k, j, i */
+static const int lay_fold[OR_LAYOUTS][2] = {{8, 8}, {4, 16}, {8, 8}};        /* j, i */
+
+int or_layout_extent(int lay, int axis) { /* axis 0 = k, 1 = j, 2 = i */
+  if (lay < 0 || lay >= OR_LAYOUTS || axis < 0 || axis > 2) return 0;
+  return lay_ext[lay][axis];
+}
+
+static inline int lay_off(int lay, int k, int j, int i) {
+  const int BJ = lay_ext[lay][1], BI = lay_ext[lay][2];
+  const int FJ = lay_fold[lay][0], FI = lay_fold[lay][1];
+  const int vec = (k * (BJ / FJ) + j / FJ) * (BI / FI) + i / FI;
+  return vec * (FJ * FI) + (j % FJ) * FI + i % FI;
+}
+
+int or_layout_offset(int lay, int k, int j, int i) {
+  if (lay < 0 || lay >= OR_LAYOUTS) return -1;
+  return lay_off(lay, k, j, i);
+}
+
+/* seeded fill of a layout's brick grid: nb interior bricks per axis
+ * (contiguous-first), `ghost` (0 or 1) ghost layers; cell (x, y, z) of the
+ * n = nb * extent interior gets or_u01(seed, x + n0*(y + n1*z)), the value
+ * or_fill_uniform gives the same cell. */
+void or_fill_uniform_layout(int lay, int dt, void *dat, size_t step, const long nb[3],
+                            int ghost, uint64_t seed, double lo, double hi) {
+  const int BK = lay_ext[lay][0], BJ = lay_ext[lay][1], BI = lay_ext[lay][2];
+  const long n0 = nb[0] * BI, n1 = nb[1] * BJ, n2 = nb[2] * BK;
+  const long S0 = nb[0] + 2 * ghost, S1 = nb[1] + 2 * ghost;
+#pragma omp parallel for collapse(2)
+  for (long z = 0; z < n2; ++z)
+    for (long y = 0; y < n1; ++y)
+      for (long x = 0; x < n0; ++x) {
+        uint64_t lin = (uint64_t)x + (uint64_t)n0 * ((uint64_t)y + (uint64_t)n1 * (uint64_t)z);
+        double v = lo + (hi - lo) * or_u01(seed, lin);
+        size_t b = (size_t)((x / BI + ghost) + S0 * ((y / BJ + ghost) + S1 * (z / BK + ghost)));
+        st_from_f64(dt, dat, b * step + (size_t)lay_off(lay, (int)(z % BK), (int)(y % BJ),
+                                                        (int)(x % BI)), v);
+      }
+}
+
+/* the interior of a layout's brick grid (as above) to a dense FP64 array
+ * (n2, n1, n0), x fastest */
+void or_to_dense_layout(int lay, int dt, const void *dat, size_t step, const long nb[3],
+                        int ghost, double *arr) {
+  const int BK = lay_ext[lay][0], BJ = lay_ext[lay][1], BI = lay_ext[lay][2];
+  const long n0 = nb[0] * BI, n1 = nb[1] * BJ, n2 = nb[2] * BK;
+  const long S0 = nb[0] + 2 * ghost, S1 = nb[1] + 2 * ghost;
+#pragma omp parallel for collapse(2)
+  for (long z = 0; z < n2; ++z)
+    for (long y = 0; y < n1; ++y)
+      for (long x = 0; x < n0; ++x) {
+        size_t b = (size_t)((x / BI + ghost) + S0 * ((y / BJ + ghost) + S1 * (z / BK + ghost)));
+        arr[(z * n1 + y) * n0 + x] =
+            ld_as_f64(dt, dat, b * step + (size_t)lay_off(lay, (int)(z % BK), (int)(y % BJ),
+                                                         (int)(x % BI)));
+      }
+}
+
+#define OR_LBOXMAX (12 * 16 * 24) /* largest (BK+2R)(BJ+2R)(BI+2R): R = 4 on 4x8x16 */
+
+/* or_brick_sweep for any layout: the same per-point evaluation (and so the
+ * same bits per cell), the neighbourhood gathered through the adjacency
+ * with the layout's extents (neighbour digit (x + B) / B per axis, as the
+ * accessor's, brick.h:235-246) and element offsets. */
+void or_brick_sweep_layout(int lay, int st, int dt, int mode, const uint32_t *adj,
+                           const uint32_t *list, uint32_t nlist, const void *in,
+                           size_t in_step, void *out, size_t out_step,
+                           const double *coeff, double theta, int cond_mode) {
+  plan_t pl;
+  make_plan(&pl, st, mode, coeff, theta, cond_mode);
+  const int R = pl.R, P = pl.P;
+  const int BK = lay_ext[lay][0], BJ = lay_ext[lay][1], BI = lay_ext[lay][2];
+  const int Pj = BJ + 2 * R, Pi = BI + 2 * R, Pk = BK + 2 * R;
+  /* re-base the tap offsets on this box */
+  for (int q = 0; q < pl.ntap; ++q) {
+    const int bo = pl.boff[q] + (R * P + R) * P + R;
+    const int di = bo % P - R, dj = (bo / P) % P - R, dk = bo / (P * P) - R;
+    pl.boff[q] = (dk * Pj + dj) * Pi + di;
+  }
+#pragma omp parallel
+  {
+    double box[OR_LBOXMAX];
+    float boxf[OR_LBOXMAX];
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nlist; ++l) {
+      const uint32_t b = list[l];
+      const uint32_t *ab = adj + (size_t)b * 27;
+      for (int k = -R; k < BK + R; ++k)
+        for (int j = -R; j < BJ + R; ++j)
+          for (int i = -R; i < BI + R; ++i) {
+            const int kd = (k + BK) / BK, jd = (j + BJ) / BJ, id = (i + BI) / BI;
+            const int e = lay_off(lay, (k + BK) % BK, (j + BJ) % BJ, (i + BI) % BI);
+            box[((k + R) * Pj + (j + R)) * Pi + (i + R)] =
+                ld_as_f64(dt, in, (size_t)ab[kd * 9 + jd * 3 + id] * in_step + (size_t)e);
+          }
+      if (dt != OR_F64)
+        for (int q = 0; q < Pk * Pj * Pi; ++q) boxf[q] = (float)box[q]; /* exact */
+      for (int k = 0; k < BK; ++k)
+        for (int j = 0; j < BJ; ++j)
+          for (int i = 0; i < BI; ++i) {
+            const int c = ((k + R) * Pj + (j + R)) * Pi + (i + R);
+            const size_t oi = (size_t)b * out_step + (size_t)lay_off(lay, k, j, i);
+            if (dt == OR_F64) {
+              ((double *)out)[oi] = eval_f64(&pl, box + c);
+            } else {
+              const float v = eval_f32(&pl, boxf + c);
+              if (dt == OR_F32) ((float *)out)[oi] = v;
+              else ((uint16_t *)out)[oi] = f32_to_bf16(v);
+            }
+          }
+    }
+  }
+}
+
 /* The reference's accessor loop for the 7-point star, restated literally:
  * stencils/3axis.cpp:51-66, every access through the adjacency list
  * (brick.h:235-246).  Used as the CPU baseline and to cross-check the

@@ -39,12 +39,12 @@ struct InfoView {
 };
 
 template <typename B>
-void offsets_of(B &br, unsigned b, long *out) {
+void offsets_of(B &br, unsigned b, long *out, int BK = 8, int BJ = 8, int BI = 8) {
   const bElem *base = br.dat;
   long n = 0;
-  for (int k = 0; k < 8; ++k)
-    for (int j = 0; j < 8; ++j)
-      for (int i = 0; i < 8; ++i) out[n++] = (long)(&br[b][k][j][i] - base);
+  for (int k = 0; k < BK; ++k)
+    for (int j = 0; j < BJ; ++j)
+      for (int i = 0; i < BI; ++i) out[n++] = (long)(&br[b][k][j][i] - base);
 }
 
 }  // namespace
@@ -66,6 +66,8 @@ int ref_init_grid(const long dims[3], unsigned *grid_out, unsigned *adj_out) {
 // brick b, for the 8x8x8 brick in the given fold:
 //   0: Dim<8,8> (HIP/AVX-512)   1: Dim<8> (SSE-style)   2: Dim<2,4>
 //   3: Dim<4,8> (CUDA)          4: Dim<2,2> (AVX2)
+// and for the 4x8x16 brick (k, j, i extents 4, 8, 16):
+//   5: Dim<4,16>                6: Dim<8,8>
 int ref_fold_offsets(int fold, unsigned *adj, unsigned nbricks, unsigned b,
                      size_t step, long *out) {
   InfoView iv(adj, nbricks);
@@ -77,6 +79,8 @@ int ref_fold_offsets(int fold, unsigned *adj, unsigned nbricks, unsigned b,
     case 2: { Brick<Dim<8, 8, 8>, Dim<2, 4>> br(&iv.info, st, 0); offsets_of(br, b, out); break; }
     case 3: { Brick<Dim<8, 8, 8>, Dim<4, 8>> br(&iv.info, st, 0); offsets_of(br, b, out); break; }
     case 4: { Brick<Dim<8, 8, 8>, Dim<2, 2>> br(&iv.info, st, 0); offsets_of(br, b, out); break; }
+    case 5: { Brick<Dim<4, 8, 16>, Dim<4, 16>> br(&iv.info, st, 0); offsets_of(br, b, out, 4, 8, 16); break; }
+    case 6: { Brick<Dim<4, 8, 16>, Dim<8, 8>> br(&iv.info, st, 0); offsets_of(br, b, out, 4, 8, 16); break; }
     default: return -1;
   }
   return 0;
@@ -90,6 +94,21 @@ long ref_neighbor_offset(unsigned *adj, unsigned nbricks, unsigned b,
   std::vector<double> dummy(1);
   auto st = wrap_storage(dummy.data(), nbricks, step);
   Brick<Dim<8, 8, 8>, Dim<8, 8>> br(&iv.info, st, 0);
+  return (long)(&br[b][k][j][i] - br.dat);
+}
+
+// The same through the 4x8x16 brick: layout 1 = Dim<4,16> fold,
+// layout 2 = Dim<8,8> fold; (k, j, i) anywhere in the 3x3x3 neighbourhood.
+long ref_neighbor_offset_layout(int lay, unsigned *adj, unsigned nbricks, unsigned b,
+                                size_t step, int k, int j, int i) {
+  InfoView iv(adj, nbricks);
+  std::vector<double> dummy(1);
+  auto st = wrap_storage(dummy.data(), nbricks, step);
+  if (lay == 1) {
+    Brick<Dim<4, 8, 16>, Dim<4, 16>> br(&iv.info, st, 0);
+    return (long)(&br[b][k][j][i] - br.dat);
+  }
+  Brick<Dim<4, 8, 16>, Dim<8, 8>> br(&iv.info, st, 0);
   return (long)(&br[b][k][j][i] - br.dat);
 }
 

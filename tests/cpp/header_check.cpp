@@ -3,7 +3,8 @@
 // Prints, for the CPU tests to compare with the oracle and oracle/_ref:
 //   grid <d0> <d1> <d2> <periodic>: the grid numbering and adjacency;
 //   fold <id>: element offsets of every (k,j,i) of brick 13 of a 3x3x3 grid
-//              in fold 0: Dim<8,8>  1: Dim<8>  2: Dim<2,4>  3: Dim<4,8>  4: Dim<2,2>;
+//              in fold 0: Dim<8,8>  1: Dim<8>  2: Dim<2,4>  3: Dim<4,8>  4: Dim<2,2>,
+//              and of the 4x8x16 brick in fold 5: Dim<4,16>  6: Dim<8,8>;
 //   neighbor: offsets of (k,j,i) in [-8,16)^3 on a stride of 5, brick 13;
 //   copy: copyToBrick then copyFromBrick round trip and compareBrick.
 #include <cstdio>
@@ -15,10 +16,10 @@
 #include "bricksetup.h"
 
 template <typename B>
-void print_offsets(B &br, unsigned b) {
-  for (int k = 0; k < 8; ++k)
-    for (int j = 0; j < 8; ++j)
-      for (int i = 0; i < 8; ++i) std::printf("%ld\n", (long)(&br[b][k][j][i] - br.dat));
+void print_offsets(B &br, unsigned b, int BK = 8, int BJ = 8, int BI = 8) {
+  for (int k = 0; k < BK; ++k)
+    for (int j = 0; j < BJ; ++j)
+      for (int i = 0; i < BI; ++i) std::printf("%ld\n", (long)(&br[b][k][j][i] - br.dat));
 }
 
 int main(int argc, char **argv) {
@@ -43,6 +44,8 @@ int main(int argc, char **argv) {
       case 2: { Brick<Dim<8, 8, 8>, Dim<2, 4>> br(&info, st, 0); print_offsets(br, 13); break; }
       case 3: { Brick<Dim<8, 8, 8>, Dim<4, 8>> br(&info, st, 0); print_offsets(br, 13); break; }
       case 4: { Brick<Dim<8, 8, 8>, Dim<2, 2>> br(&info, st, 0); print_offsets(br, 13); break; }
+      case 5: { Brick<Dim<4, 8, 16>, Dim<4, 16>> br(&info, st, 0); print_offsets(br, 13, 4, 8, 16); break; }
+      case 6: { Brick<Dim<4, 8, 16>, Dim<8, 8>> br(&info, st, 0); print_offsets(br, 13, 4, 8, 16); break; }
       default: return 2;
     }
     return 0;

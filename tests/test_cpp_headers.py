@@ -40,7 +40,7 @@ def test_grid_matches_oracle(exe, dims, periodic):
     np.testing.assert_array_equal(vals[n:].reshape(n, 27), a)
 
 
-@pytest.mark.parametrize("fold", range(5))
+@pytest.mark.parametrize("fold", range(7))
 def test_fold_offsets_match_reference(exe, fold):
     got = np.array([int(x) for x in run(exe, "fold", fold) if x])
     if not O.ref_available():

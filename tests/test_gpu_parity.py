@@ -415,10 +415,14 @@ def test_tuner_picks_a_schedule_with_the_same_bits(gpu, name):
     g.fill_uniform(a, 2)
     before = gpu.sweeps(st, a, b, g, 3).storage.to_host()[g.interior]
     out = g.tune(st, BK_F64, steps=2, warm_s=0.05)
-    assert set(out["ms"]) == {"none", f"('{'quad' if name == '125pt' else 'column'}', 4)",
-                              f"('{'quad' if name == '125pt' else 'column'}', 8)",
-                              f"('{'quad' if name == '125pt' else 'column'}', 16)"}
-    assert str(out["best"]) in out["ms"]
+    if name == "125pt":
+        want = {"8x8x8 brick wg4", "8x8x8 quad2", "8x8x8 quad4", "8x8x8 quad8"}
+    else:
+        want = {"8x8x8 brick wg1", "8x8x8 brick wg2", "8x8x8 brick wg4", "8x8x8 brick wg8",
+                "8x8x8 column4", "8x8x8 column8", "8x8x8 column16", "8x8x8 quad4",
+                "8x8x8 quad8"}
+    assert set(out["ms"]) == want
+    assert out["best"] in out["ms"]
     g.fill_uniform(a, 2)
     after = gpu.sweeps(st, a, b, g, 3).storage.to_host()[g.interior]
     np.testing.assert_array_equal(after, before)

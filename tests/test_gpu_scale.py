@@ -227,11 +227,29 @@ def wave_error(got, exp):
 WAVES = [(0, 0, 0), (1, 2, 3), (5, 0, 7)]
 
 
+def wave_stencil(kind, seed):
+    """The stencil a closed-form wave check runs: the workload's stars
+    (positive, symmetric, sum 1), and for the cube coefficients U(0,1)
+    normalised to sum 1.  The cube's workload coefficients (U(-1,1) over
+    their L1 norm) give every low wave |lam| ~ 0.04, so after 100 sweeps the
+    wave is ~1e-140 while FP64 rounding of the input cos() puts ~1e-16 into
+    modes that decay far more slowly: the closed form then describes
+    rounding noise, not the stencil.  Positive coefficients summing to 1
+    make the low waves the slowest-decaying modes.  (The workload's cube
+    coefficients are checked against the oracle at 512^3 and over 100
+    sweeps above.)"""
+    st = make_stencil(kind, seed=seed)
+    if kind == S.BK_CUBE125:
+        c = np.array([S.u01(seed, q) for q in range(125)])
+        st = S.Stencil(kind, c / c.sum())
+    return st
+
+
 @pytest.mark.skipif(torch is None, reason="torch is the device-array plumbing of this test")
 @pytest.mark.parametrize("kind", LINEAR, ids=lambda s: NAMES[s])
 def test_plane_waves_512_fp64_100_sweeps(gpu, kind):
     nb = (64, 64, 64)
-    st = make_stencil(kind, seed=6)
+    st = wave_stencil(kind, seed=6)
     for k in WAVES:
         print(f"[case] plane wave k={k} {NAMES[kind]} FP64 512^3 100 sweeps", flush=True)
         dense = dense_wave(nb, k, BK_F64)
@@ -274,10 +292,7 @@ def test_plane_waves_1024_fp32(gpu, kind, nsweeps):
     field stays far above FP32's range limits over the sweeps; the error
     bound is 1e-5 of the field's RMS."""
     nb = (128, 128, 128)
-    st = make_stencil(kind, seed=9)
-    if kind == S.BK_CUBE125:
-        c = np.array([S.u01(9, q) for q in range(125)])
-        st = S.Stencil(kind, c / c.sum())
+    st = wave_stencil(kind, seed=9)
     for k in [(0, 0, 0), (1, 2, 3)]:
         print(f"[case] plane wave k={k} {NAMES[kind]} FP32 1024^3 {nsweeps} sweeps", flush=True)
         dense = dense_wave(nb, k, BK_F32)
