@@ -237,6 +237,8 @@ SIGNATURES = {
     "bk_fill_uniform_grid": (C.c_int, [C.c_int, vp, C.c_size_t, longp, C.c_int, C.c_uint64,
                                        C.c_double, C.c_double, vp]),
     "bk_field_stats": (C.c_int, [C.c_int, vp, C.c_size_t, vp, C.c_uint32, C.c_size_t, dblp, vp]),
+    "bk_field_stats_noalloc": (C.c_int, [C.c_int, vp, C.c_size_t, vp, C.c_uint32, C.c_size_t,
+                                         vp, vp]),
     "bk_field_compare": (C.c_int, [C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_size_t, vp,
                                    C.c_uint32, C.c_size_t, C.c_double, C.c_double, C.c_double,
                                    dblp, vp]),

@@ -86,6 +86,50 @@ int get_scratch(double **p) {
   return BK_OK;
 }
 
+}  // namespace
+
+// Pass-1 partials of bk_field_stats_noalloc: a file-scope device array, one
+// copy per GPU at its own address, held until the process ends (24 KiB).
+// Its size does not depend on the input: the first pass is grid-stride over
+// a fixed kReduceBlocks blocks, each writing its own three partials, and the
+// second pass reads all of them, so every slot is rewritten by every call
+// and nothing needs clearing.  Calls that overlap on one GPU share it and
+// race.
+constexpr size_t kStaticScratchDoubles = (size_t)bk::kReduceBlocks * 3;
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+namespace {
+
+// The array's device address on each GPU, looked up once per device with
+// that device current (its name is a host-side stand-in, not an address).
+constexpr int kMaxDevices = 64;
+struct StaticScratch {
+  std::mutex mu;
+  double *ptr[kMaxDevices] = {};
+};
+StaticScratch &static_scratch() {
+  static StaticScratch s;
+  return s;
+}
+
+int static_scratch_ptr(double **p) {
+  int dev = 0;
+  BK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(BK_ERR_ARG, "bk_field_stats_noalloc: device %d beyond the %d supported", dev,
+                kMaxDevices);
+  auto &st = static_scratch();
+  std::lock_guard<std::mutex> lock(st.mu);
+  if (!st.ptr[dev]) {
+    void *q = nullptr;
+    BK_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    if (!q) return fail(BK_ERR_HIP, "bk_field_stats_noalloc: no device address for the scratch array");
+    st.ptr[dev] = static_cast<double *>(q);
+  }
+  *p = st.ptr[dev];
+  return BK_OK;
+}
+
 // Graph cache for bk_sweeps(use_graph): key = every argument byte, with
 // plans keyed by their ids (a plan's host handle can be reused by a later
 // plan once freed; its id cannot).  At most 64 graphs, evicted least
@@ -810,6 +854,24 @@ int bk_field_stats(int dtype, const void *d_dat, size_t step, const uint32_t *d_
                                 S(s)));
   BK_HIP(hipMemcpyAsync(h_out, buf, 3 * sizeof(double), hipMemcpyDeviceToHost, S(s)));
   BK_HIP(hipStreamSynchronize(S(s)));
+  return BK_OK;
+}
+
+int bk_field_stats_noalloc(int dtype, const void *d_dat, size_t step, const uint32_t *d_list,
+                           uint32_t nlist, size_t brick_elems, double *d_out, bk_stream s) {
+  static_assert((size_t)bk::kReduceBlocks * 3 <= kStaticScratchDoubles,
+                "pass-1 partials exceed the static scratch array");
+  if (!valid_dtype(dtype))
+    return fail(BK_ERR_ARG, "bk_field_stats_noalloc: unknown dtype %d", dtype);
+  if (!d_out) return fail(BK_ERR_ARG, "bk_field_stats_noalloc: null output");
+  if (nlist && (!d_dat || !d_list))
+    return fail(BK_ERR_ARG, "bk_field_stats_noalloc: null pointer");
+  if (brick_elems == 0 || brick_elems > step)
+    return fail(BK_ERR_ARG, "bk_field_stats_noalloc: brick_elems must be in [1, step]");
+  double *buf = nullptr;
+  int rc = static_scratch_ptr(&buf);
+  if (rc != BK_OK) return rc;
+  BK_HIP(bk::launch_field_stats(dtype, d_dat, step, d_list, nlist, brick_elems, buf, d_out, S(s)));
   return BK_OK;
 }
 

@@ -594,6 +594,18 @@ int bk_fill_uniform_layout(int layout, int dtype, void *d_dat, size_t step,
 int bk_field_stats(int dtype, const void *d_dat, size_t step,
                    const uint32_t *d_list, uint32_t nlist, size_t brick_elems,
                    double h_out[3], bk_stream s);
+/* bk_field_stats without any allocation: no counterpart in the reference
+ * (it has no GPU reductions); for callers that cannot allocate device memory
+ * -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The same kernels and the same results as bk_field_stats, but
+ * the pass-1 partials live in a static 24 KiB device array of the library
+ * (one per GPU, whatever the input size) and the three results are written
+ * to device memory d_out[3], stream-ordered on s with no host wait.  Calls
+ * that overlap on one GPU share that array and race: order them on one
+ * stream (or wait between them). */
+int bk_field_stats_noalloc(int dtype, const void *d_dat, size_t step,
+                           const uint32_t *d_list, uint32_t nlist,
+                           size_t brick_elems, double *d_out, bk_stream s);
 /* FP64 comparison of field a against reference field r (dtypes may differ)
  * over d_list:  out[0] = max |a-r|,  out[1] = max |a-r| / max(|r|, floor),
  * out[2] = sum (a-r)^2,  out[3] = sum r^2,
