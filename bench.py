@@ -370,6 +370,17 @@ def serve_bench(bk, S, args):
     out["saturating_tuned" if alt else "saturating_default_schedules"] = sat2
     print(f"[bench] serving saturating ({'tuned' if alt else 'default'} schedules): {sat2}",
           file=sys.stderr, flush=True)
+    # and with the stars one brick per wave (the batched schedule before the
+    # 2x2 brick blocks)
+    srv3 = SV.Server(hbm_budget=200e9, chunk=16, ghost=serve_ghost, star_plan=None)
+    reqs4 = SV.workload(args.serve_requests // 2, rate=args.serve_rate, seed=77)
+    srv3.prewarm(reqs4)
+    st4 = srv3.serve(reqs4, realtime=False)
+    sat3 = st4.summary()
+    sat3["efficiency_vs_single_grid"] = round(single_time / st4.elapsed, 4)
+    out["saturating_stars_per_brick"] = sat3
+    print(f"[bench] serving saturating (stars one brick per wave): {sat3}", file=sys.stderr,
+          flush=True)
     return out
 
 
@@ -418,6 +429,10 @@ def main():
     # periodic ghost bricks of its output (the input's ghosts are refreshed
     # once first): bk_sweeps_fused, the same bits as a separate refresh
     # before every sweep (that form is by_stencil's 7pt_f64_separate_refresh)
+    pl7 = g.auto_plan(st7.kind, bk.BK_F64)
+    sched7 = ("one brick per wave" if pl7 is None else
+              f"2x2 brick blocks ({pl7.shape} plan, {pl7.kcol} level(s) per unit), one "
+              f"workgroup of four waves per block")
     ms_step = time_sweeps(bk, st7, a, b, g, stream, args.steps, args.warmup, fused=True,
                           warm_s=args.warm_s)
     # the dominant kernel alone, for the roofline
@@ -510,15 +525,19 @@ def main():
         # sweep): every brick layout (8x8x8, 4x8x16 in the 4x16 and the 8x8
         # fold) and every schedule of each, fastest first
         cur.clear()
-        for sname in ("7pt", "13pt", "25pt", "125pt"):
+        for sname, dt, layouts in (("7pt", bk.BK_F64, (0, 1, 2)), ("13pt", bk.BK_F64, (0, 1, 2)),
+                                   ("25pt", bk.BK_F64, (0, 1, 2)), ("125pt", bk.BK_F64, (0, 1, 2)),
+                                   ("7pt", bk.BK_F32, (0,)), ("25pt", bk.BK_F32, (0,)),
+                                   ("7pt", bk.BK_BF16, (0,))):
             st = S.Stencil.make(sname, seed=1)
             t0 = time.perf_counter()
-            res = bk.tune(st, (n, n, n), bk.BK_F64, ghost=1, steps=10, rounds=2,
-                          warm_s=args.warm_s / 2, stream=stream)
-            tuned[sname] = {"best": bk.cand_name(res[0]),
-                            "ms_per_step": {bk.cand_name(c): round(c.ms, 4) for c in res},
-                            "tune_s": round(time.perf_counter() - t0, 2)}
-            print(f"[bench] tune {sname}: {tuned[sname]}", file=sys.stderr, flush=True)
+            res = bk.tune(st, (n, n, n), dt, ghost=1, steps=10, rounds=2,
+                          warm_s=args.warm_s / 2, layouts=layouts, stream=stream)
+            key = f"{sname}_{'f64' if dt == bk.BK_F64 else 'f32' if dt == bk.BK_F32 else 'bf16'}"
+            tuned[key] = {"best": bk.cand_name(res[0]),
+                          "ms_per_step": {bk.cand_name(c): round(c.ms, 4) for c in res},
+                          "tune_s": round(time.perf_counter() - t0, 2)}
+            print(f"[bench] tune {key}: {tuned[key]}", file=sys.stderr, flush=True)
 
     prec = {}
     if not args.no_precision:
@@ -581,6 +600,7 @@ def main():
         "config": {"workload": f"7-point star FP64, {n}^3 interior + 8-cell ghost, 8x8x8 bricks "
                                f"(fold 8x8), periodic ghost-brick refresh every sweep (fused into "
                                f"the sweep's stores), ping-pong sweeps", "interior_cells": pts,
+                   "schedule": sched7,
                    "bricks": g.nbricks, "ghost_bricks": g.npairs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -62,6 +62,7 @@ class bk_serve_config(C.Structure):
         ("pipeline", C.c_int),
         ("tune", C.c_int),
         ("slab_bytes", C.c_size_t),
+        ("star_plan", C.c_int),
     ]
 
 

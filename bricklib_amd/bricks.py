@@ -330,6 +330,34 @@ class Plan:
 # ---------------------------------------------------------------------------
 # periodic brick grid with one ghost layer
 # ---------------------------------------------------------------------------
+# Default schedule per (stencil, dtype): (plan shape, levels), None = one
+# brick per wave.  From warmed timings of every schedule at 512^3 on one
+# MI355X (ms per step with the refresh fused into the sweep / with a separate
+# refresh kernel / kernel alone):
+#   stars and cond: 2x2 column quads, whose four waves share one LDS tile
+#   of the block, so each wave fetches two of its four in-plane faces --
+#   FP64 7-point quad1 0.420 / 0.441 / 0.396 against one brick per wave
+#   0.466 / 0.478 / 0.436; 13-point quad1 0.440 / 0.457 against 0.514 /
+#   0.516; cond quad1 0.422 / 0.441 against 0.466 / 0.481; 25-point quad2
+#   0.505 / 0.510 against k-marching columns of 8 0.598 (separate); FP32
+#   7-point quad2 0.235 / 0.241 against 0.257 / 0.269, 13-point and cond
+#   quad2, 25-point quad4 0.287 / 0.289 against 0.390 / 0.366.  One level
+#   (no k-march) wins for FP64 radius 1-2: four times the units of quad4,
+#   and the k-faces a march would save are a quarter of what the shared
+#   tile saves.  BF16 7-point: one brick per wave (0.157 with a separate
+#   refresh against quad4 0.164); other BF16 stars follow FP32's choice.
+#   cube: quads of 4 levels (1-3% ahead of 8 levels at 512^3, whose
+#   smaller halo share is outweighed by the shorter tail of more units).
+DEFAULT_PLANS = {
+    (0, BK_F64): ("quad", 1), (1, BK_F64): ("quad", 1), (4, BK_F64): ("quad", 1),
+    (2, BK_F64): ("quad", 2),
+    (0, BK_F32): ("quad", 2), (1, BK_F32): ("quad", 2), (4, BK_F32): ("quad", 2),
+    (2, BK_F32): ("quad", 4),
+    (1, BK_BF16): ("quad", 2), (4, BK_BF16): ("quad", 2), (2, BK_BF16): ("quad", 4),
+    (3, BK_F64): ("quad", 4), (3, BK_F32): ("quad", 4), (3, BK_BF16): ("quad", 4),
+}
+
+
 class PeriodicGrid:
     """n = 8*nb interior cells per axis with periodic boundaries, in one of
     two layouts, all tables in HBM:
@@ -349,7 +377,7 @@ class PeriodicGrid:
     nb[0]*BI x nb[1]*BJ x nb[2]*BK cells.  Each cell gets the same value in
     every layout."""
 
-    def __init__(self, nb, stream=None, ghost: int = 1, tile: int = 32, layout: int = 0):
+    def __init__(self, nb, stream=None, ghost: int = 1, tile=(32, 16), layout: int = 0):
         self.nb = tuple(int(x) for x in nb)
         if ghost not in (0, 1):
             raise ValueError("ghost must be 0 or 1")
@@ -381,7 +409,9 @@ class PeriodicGrid:
         # tiles (tile=T: T x T), each swept through all k before the next --
         # neighbours in k are then Ti*Tj list positions apart instead of a
         # whole brick plane (T = 32 measured 4-6% faster for the 7-point star
-        # at 512^3 than grid order).  Every brick is computed the same way in
+        # at 512^3 than grid order; for the 2x2 block schedules, whose plans
+        # follow this order, 32 x 16 is 0-3% ahead of 32 x 32 and 6-10% ahead
+        # of whole planes or 8 x 8).  Every brick is computed the same way in
         # any order.
         self.tile = (tuple(int(x) for x in tile) if isinstance(tile, (tuple, list))
                      else ((int(tile), int(tile)) if tile else 0))
@@ -439,26 +469,25 @@ class PeriodicGrid:
             self._d_every = DeviceBuffer.from_array(np.arange(self.nbricks, dtype=np.uint32))
         return self._d_every
 
-    def auto_plan(self, stencil_kind: int):
-        """Default schedule per stencil: the tuned one when tune() ran for
-        it on this grid, else, from warmed measurements at 512^3 on MI355X,
-        2x2 column quads of 4 levels for the cube (1-3% ahead of 8 levels,
-        whose smaller halo share is outweighed by the shorter tail of more,
-        shorter units), k-marching columns of 8 bricks for the radius-4
-        star, and one brick per wave (no plan) for the radius-1 and radius-2
-        stars.  The 4x8x16 layouts always run one brick per wave."""
-        from ._lib import BK_CUBE125, BK_STAR25
+    def auto_plan(self, stencil_kind: int, dtype: int = BK_F64):
+        """Default schedule per stencil and dtype: the tuned one when tune()
+        ran for the stencil on this grid, else DEFAULT_PLANS (measured on
+        MI355X at 512^3).  The 4x8x16 layouts always run one brick per
+        wave."""
         tuned = self._tuned.get(int(stencil_kind))
         if tuned is not None:
             return None if tuned.plan_shape == 0 else self.plan(
                 "quad" if tuned.plan_shape == _lib.BK_PLAN_QUAD else "column", tuned.kcol)
         if self.brick_layout != 0:
             return None
-        if stencil_kind == BK_CUBE125:
-            return self.plan("quad", 4)
-        if stencil_kind == BK_STAR25:
-            return self.plan("column", 8)
-        return None
+        choice = DEFAULT_PLANS.get((int(stencil_kind), int(dtype)))
+        if choice is None:
+            return None
+        if int(stencil_kind) == _lib.BK_CUBE125 and self.interior.size > (1 << 20):
+            # large grids: 8 levels (1024^3 FP64: 4.76 ms per sweep, the same
+            # per cell as 512^3 with 4 levels, against 5.0-6.2 ms with 4)
+            choice = ("quad", 8)
+        return self.plan(*choice)
 
     def auto_wg_bricks(self, stencil_kind: int) -> int:
         """Bricks per workgroup of the per-brick kernels: the tuned value,
@@ -538,12 +567,12 @@ class PeriodicGrid:
                 "rms_abs": float(np.sqrt(out[2] / n)), "bad": int(out[4])}
 
 
-def _resolve_plan(plan, st: Stencil, grid: PeriodicGrid):
+def _resolve_plan(plan, st: Stencil, grid: PeriodicGrid, dtype: int = BK_F64):
     if plan is None:
         return None
     if isinstance(plan, str):
         if plan == "auto":
-            return grid.auto_plan(st.kind)
+            return grid.auto_plan(st.kind, dtype)
         return grid.plan(plan)
     return plan
 
@@ -571,7 +600,7 @@ def _desc(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, coeff_keep: l
     d.h_coeff = coeff.ctypes.data_as(_lib.dblp)
     d.theta = float(st.theta)
     d.cond_mode = int(st.cond_mode)
-    pl = _resolve_plan(plan, st, grid)
+    pl = _resolve_plan(plan, st, grid, inp.dtype)
     d.plan = pl.handle if pl is not None else None
     coeff_keep.append(pl)
     d.layout = getattr(grid, "brick_layout", 0)
@@ -601,8 +630,8 @@ def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
     per ghost_iters sweeps and the sweeps in between over every brick of the
     padded grid (bk_sweeps_deep), or, with fused=True, the refresh folded
     into the sweeps' stores (bk_sweeps_fused; stars and cond, one brick per
-    wave).  Same result in every form.  Returns the field holding the
-    result."""
+    wave or by quad plans).  Same result in every form.  Returns the field
+    holding the result."""
     keep: list = []
     d = _desc(st, b, a, grid, keep, plan=plan, wg_bricks=wg_bricks)
     pairs = grid.d_pairs.ptr if (refresh and grid.d_pairs is not None) else None
@@ -621,7 +650,7 @@ def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
         # sweeps over every brick: column plans as for the interior; the
         # cube per brick (quad plans cannot take the outer layer of a
         # non-periodic grid, bk_plan_create rejects them)
-        ip = _resolve_plan(plan, st, grid)
+        ip = _resolve_plan(plan, st, grid, a.dtype)
         ap = (grid.plan(ip.shape, ip.kcol, every=True)
               if ip is not None and ip.shape != "quad" else None)
         keep.append(ap)

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 
 #include "bk_api.h"
@@ -25,14 +26,15 @@ int plan_fail(int code, const char *msg);
 }
 
 namespace {
-// sweep groups: (stencil, dtype, schedule), schedule 1 for star requests
-// that come with a column plan (k-marching units instead of single bricks)
-constexpr int kStencils = 5, kDtypes = 3, kGroups = kStencils * kDtypes * 2;
-static_assert(kGroups <= 32, "the non-empty groups form a 32-bit mask");
+// sweep groups: (stencil, dtype, schedule); the schedule of a star request
+// is 0 without a plan (one brick per wave), 1 with a column plan (k-marching
+// units), 2 with a quad plan (2x2 blocks of k-columns per workgroup)
+constexpr int kStencils = 5, kDtypes = 3, kScheds = 3, kGroups = kStencils * kDtypes * kScheds;
+static_assert(kGroups <= 64, "the non-empty groups form a 64-bit mask");
 constexpr size_t kCoeffSlot = 125;  // doubles per request in the coefficient pool
 
-constexpr int group_of(int stencil, int dtype, bool columns) {
-  return (stencil * kDtypes + dtype) * 2 + (columns ? 1 : 0);
+constexpr int group_of(int stencil, int dtype, int sched) {
+  return (stencil * kDtypes + dtype) * kScheds + sched;
 }
 }  // namespace
 
@@ -45,14 +47,14 @@ struct bk_batch_s {
   bk::BatchTable *h_ref[kDtypes] = {};
   double *d_coeff = nullptr;
   double *h_coeff = nullptr;
-  uint32_t sweep_mask = 0;  // non-empty sweep groups
+  uint64_t sweep_mask = 0;  // non-empty sweep groups
   uint32_t ref_mask = 0;    // non-empty refresh tables
   hipEvent_t staged = nullptr;
   hipStream_t capture = nullptr;
-  // captured runs keyed by (tick count, refresh and sweep group masks); an
+  // captured runs keyed by (tick count, sweep and refresh group masks); an
   // exec is destroyed only once its last launch has completed, so a caller
   // may queue the next run while the previous one is still on the device
-  bk::ExecCache<std::pair<uint32_t, uint64_t>> graphs{64};
+  bk::ExecCache<std::tuple<uint32_t, uint64_t, uint32_t>> graphs{64};
   std::mutex mu;
 };
 
@@ -83,13 +85,14 @@ hipError_t enqueue_run(bk_batch_s *b, uint32_t nticks, hipStream_t s) {
       }
     for (int g = 0; g < kGroups; ++g)
       if (b->sweep_mask >> g & 1u) {
-        const int stencil = g / (2 * kDtypes), dtype = (g / 2) % kDtypes;
-        const bool columns = (g & 1) != 0;
+        const int stencil = g / (kScheds * kDtypes), dtype = (g / kScheds) % kDtypes;
+        const int sched = g % kScheds;
         hipError_t e =
             stencil == BK_CUBE125
                 ? bk::launch_cube_batched(dtype, b->d_sweep[g], t, s)
-                : (columns ? bk::launch_star_columns_batched(stencil, dtype, b->d_sweep[g], t, s)
-                           : bk::launch_star_batched(stencil, dtype, b->d_sweep[g], t, s));
+                : sched == 1 ? bk::launch_star_columns_batched(stencil, dtype, b->d_sweep[g], t, s)
+                : sched == 2 ? bk::launch_star_quads_batched(stencil, dtype, b->d_sweep[g], t, s)
+                             : bk::launch_star_batched(stencil, dtype, b->d_sweep[g], t, s);
         if (e != hipSuccess) return e;
       }
   }
@@ -193,8 +196,9 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
     } else if (q.plan) {
       int shape = 0;
       if (bk_plan_query(q.plan, nullptr, nullptr, &shape, nullptr) != BK_OK ||
-          shape != BK_PLAN_COLUMN)
-        return fail(BK_ERR_ARG, "bk_batch_set: a star request's plan must be a BK_PLAN_COLUMN plan");
+          (shape != BK_PLAN_COLUMN && shape != BK_PLAN_QUAD))
+        return fail(BK_ERR_ARG, "bk_batch_set: a star request's plan must be a BK_PLAN_COLUMN or "
+                    "BK_PLAN_QUAD plan");
     } else if (q.nlist && !q.d_list) {
       return fail(BK_ERR_ARG, "bk_batch_set: null brick list");
     }
@@ -207,8 +211,9 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
   for (int d = 0; d < kDtypes; ++d) b->h_ref[d]->prefix[0] = 0;
   for (uint32_t r = 0; r < n; ++r) {
     const bk_batch_req &q = reqs[r];
-    const bool columns = q.stencil != BK_CUBE125 && q.plan != nullptr;
-    const int g = group_of(q.stencil, q.dtype, columns);
+    const int sched = (q.stencil == BK_CUBE125 || !q.plan) ? 0
+                      : bk::plan_view(q.plan).shape == BK_PLAN_QUAD ? 2 : 1;
+    const int g = group_of(q.stencil, q.dtype, sched);
     double *slot = b->h_coeff + (size_t)r * kCoeffSlot;
     const int nc = bk_stencil_ncoeff(q.stencil);
     if (q.dtype == BK_F64) {
@@ -228,7 +233,7 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
     sg.ticks = q.ticks;
     sg.theta = q.theta;
     sg.cond_mode = q.cond_mode;
-    if (q.stencil == BK_CUBE125 || columns) {
+    if (q.stencil == BK_CUBE125 || sched != 0) {
       const bk::PlanView P = bk::plan_view(q.plan);
       sg.items = P.anchor;
       sg.len = P.len;
@@ -259,7 +264,7 @@ int bk_batch_set(bk_batch b, const bk_batch_req *reqs, uint32_t n, bk_stream str
   for (int g = 0; g < kGroups; ++g) {
     bk::BatchTable *t = b->h_sweep[g];
     t->nsegs = count[g];
-    if (count[g] && t->prefix[count[g]]) b->sweep_mask |= 1u << g;
+    if (count[g] && t->prefix[count[g]]) b->sweep_mask |= 1ull << g;
     BKB_HIP(hipMemcpyAsync(b->d_sweep[g], t, table_head_bytes(), hipMemcpyHostToDevice, s));
     if (count[g])
       BKB_HIP(hipMemcpyAsync(&b->d_sweep[g]->seg[0], &t->seg[0], sizeof(bk::BatchSeg) * count[g],
@@ -290,8 +295,7 @@ int bk_batch_run(bk_batch b, uint32_t nticks, int use_graph, bk_stream stream) {
     BKB_HIP(enqueue_run(b, nticks, s));
     return BK_OK;
   }
-  const std::pair<uint32_t, uint64_t> key{nticks,
-                                          ((uint64_t)b->ref_mask << 32) | (uint64_t)b->sweep_mask};
+  const std::tuple<uint32_t, uint64_t, uint32_t> key{nticks, b->sweep_mask, b->ref_mask};
   if (!b->graphs.find(key)) {
     BKB_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueue_run(b, nticks, b->capture);

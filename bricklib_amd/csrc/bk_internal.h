@@ -105,6 +105,8 @@ hipError_t launch_star_batched(int stencil, int dtype, const BatchTable *tab, ui
                                hipStream_t s);
 hipError_t launch_star_columns_batched(int stencil, int dtype, const BatchTable *tab,
                                        uint32_t tick, hipStream_t s);
+hipError_t launch_star_quads_batched(int stencil, int dtype, const BatchTable *tab,
+                                     uint32_t tick, hipStream_t s);
 hipError_t launch_cube_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
 hipError_t launch_ghost_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
 

@@ -771,10 +771,12 @@ int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off, const uin
   if (nsweeps < 0) return fail(BK_ERR_ARG, "bk_sweeps_fused: negative sweep count");
   if (npairs && !d_pairs) return fail(BK_ERR_ARG, "bk_sweeps_fused: null pair list");
   if (!d_img_off || !d_img) return fail(BK_ERR_ARG, "bk_sweeps_fused: null ghost-image table");
-  if (d->stencil == BK_CUBE125 || d->plan || d->layout != BK_LAYOUT_8x8x8)
+  if (d->stencil == BK_CUBE125 || d->layout != BK_LAYOUT_8x8x8 ||
+      (d->plan && bk::plan_view(d->plan).shape != BK_PLAN_QUAD))
     return fail(BK_ERR_UNSUPPORTED,
-                "bk_sweeps_fused: the fused refresh runs in the per-brick star/cond kernel "
-                "(8x8x8 bricks, no plan, no cube); use bk_sweeps");
+                "bk_sweeps_fused: the fused refresh runs in the star/cond kernels over 8x8x8 "
+                "bricks one per wave or by quad plans (no column plan, no cube); use "
+                "bk_sweeps");
   return sweeps_impl(d, d_pairs, npairs, nsweeps, 1, d->d_list, d->nlist, d->plan, d_img_off,
                      d_img, use_graph, S(s));
 }

@@ -22,8 +22,8 @@
 // bk_tune_sweep found fastest (among those the batched kernels run) for
 // its stencil, dtype and size class -- brick counts binned by powers of
 // two, each class timed once on a cubic grid of that size -- otherwise by
-// the fixed defaults (2x2 column quads of 4 levels for the cube, one brick
-// per wave for the stars).
+// the fixed defaults (2x2 column quads of 4 levels for the cube; for the
+// stars the configured star_plan).
 //
 // The reference has no serving loop; its closest precedent is the strong
 // scaling driver's one launch over many subdomains (strong/main.cu:85-99).
@@ -236,6 +236,15 @@ int schedule_of(bk_server_s *s, const Req &r, int *shape, uint32_t *kcol) {
   if (st == BK_CUBE125) {
     *shape = BK_PLAN_QUAD;
     *kcol = 4;
+  } else if (s->cfg.star_plan == BK_PLAN_QUAD) {
+    // levels per unit as the single-grid defaults: FP64 radius 1-2 one
+    // level, the 25-point two; FP32/BF16 two, the 25-point four
+    *shape = BK_PLAN_QUAD;
+    const bool f64 = r.spec.dtype == BK_F64;
+    *kcol = st == BK_STAR25 ? (f64 ? 2 : 4) : (f64 ? 1 : 2);
+  } else if (s->cfg.star_plan == BK_PLAN_COLUMN) {
+    *shape = BK_PLAN_COLUMN;
+    *kcol = 8;
   } else {
     *shape = 0;
     *kcol = 0;
@@ -476,6 +485,9 @@ int bk_server_create(const bk_serve_config *cfg, bk_server *out) {
   if (cfg->ghost != 0 && cfg->ghost != 1)
     return sfail(BK_ERR_ARG, "bk_server_create: ghost must be 0 or 1");
   if (!(cfg->hbm_budget > 0.0)) return sfail(BK_ERR_ARG, "bk_server_create: budget must be > 0");
+  if (cfg->star_plan != 0 && cfg->star_plan != BK_PLAN_QUAD && cfg->star_plan != BK_PLAN_COLUMN)
+    return sfail(BK_ERR_ARG, "bk_server_create: star_plan must be 0, BK_PLAN_QUAD or "
+                 "BK_PLAN_COLUMN");
   std::unique_ptr<bk_server_s> s(new bk_server_s());
   s->cfg = *cfg;
   if (hipGetDevice(&s->device) != hipSuccess)

@@ -523,6 +523,11 @@ __global__ __launch_bounds__(64) void star_sweep_columns_batched(const BatchTabl
 // R-deep outer faces, so the faces between the members come from LDS and
 // each wave fetches only its two outer faces: half the in-plane face bytes
 // of the per-column schedules.  Same per-point arithmetic.
+// Rows of 16 + 2R cells.  (Padding the rows of radius 1 and 2 to 24 cells,
+// which puts the rows one read pass serves on distinct LDS banks, removed
+// most bank-conflict cycles but not time: 7-point FP64 397.4 against 394.6
+// us, 13-point 416.0 against 412.9, FP32 7-point 213.0 against 213.7; the
+// larger tile costs occupancy.)
 template <int R, int ES>
 struct QuadTile {
   static constexpr int P = 16 + 2 * R;  // rows and columns of the block tile
@@ -531,27 +536,30 @@ struct QuadTile {
   __device__ static constexpr int at(int k, int y, int x) { return k * PL + y * RS + x; }
 };
 
+// One quad unit: the four waves of a workgroup (member mm = wave index)
+// march L levels from the unit's anchor; mk[l] holds level l's store mask
+// (bit mm: store member mm).  Every wave of the workgroup must call it with
+// the same unit (it synchronises the workgroup once per level).  Shared by
+// the single-grid and the batched kernels.
 template <class T, int R, bool COND>
-__global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, COND> a) {
+__device__ __forceinline__ void quad_unit(const uint32_t *__restrict__ adj,
+                                          const T *__restrict__ in, T *__restrict__ out,
+                                          size_t st, size_t ost, uint32_t anchor, int L,
+                                          const uint32_t *__restrict__ mk,
+                                          const typename acc_of<T>::type (&c)[6 * R + 1],
+                                          typename acc_of<T>::type theta, int cond_mode,
+                                          const uint32_t *__restrict__ img_off,
+                                          const uint32_t *__restrict__ img,
+                                          typename acc_of<T>::type *tile) {
   using A = typename acc_of<T>::type;
   using QT = QuadTile<R, sizeof(A)>;
-  __shared__ A tile[8 * QT::PL];
-
-  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
-  if (u >= a.nunits) return;  // uniform over the workgroup
   const int mm = (int)uniform(threadIdx.x >> 6);  // member: i-offset mm & 1, j-offset mm >> 1
   const int lane = threadIdx.x & 63;
   const int j = lane >> 3, i = lane & 7;
   const int mi = mm & 1, mj = mm >> 1;
-  const int L = (int)a.len[u];
-  const size_t st = a.in_step;
-  const uint32_t *adj = a.adj;
-  A c[6 * R + 1];
-#pragma unroll
-  for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
 
   // this wave's level-0 brick
-  uint32_t b = a.anchor[u];
+  uint32_t b = anchor;
   if (mi) b = adj[(size_t)b * 27 + 14];
   if (mj) b = adj[(size_t)b * 27 + 16];
   b = uniform(b);
@@ -566,10 +574,10 @@ __global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, CON
 
   A cur[8], nxt[8], km[R];
   {
-    const T *p = a.in + (size_t)b * st;
+    const T *p = in + (size_t)b * st;
 #pragma unroll
     for (int k = 0; k < 8; ++k) cur[k] = widen(p[k * 64 + lane]);
-    const T *q = a.in + (size_t)uniform(adj[(size_t)b * 27 + 4]) * st;
+    const T *q = in + (size_t)uniform(adj[(size_t)b * 27 + 4]) * st;
 #pragma unroll
     for (int r = 0; r < R; ++r) km[r] = widen(q[(8 - R + r) * 64 + lane]);
   }
@@ -582,8 +590,8 @@ __global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, CON
     // the compute below
     A fi[R], fj[R];
     {
-      const T *pi = a.in + (size_t)uniform(ab[si]) * st;
-      const T *pj = a.in + (size_t)uniform(ab[sj]) * st;
+      const T *pi = in + (size_t)uniform(ab[si]) * st;
+      const T *pj = in + (size_t)uniform(ab[sj]) * st;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int k = q * (8 / R) + fkq;
@@ -591,7 +599,7 @@ __global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, CON
         fj[q] = widen(pj[k * 64 + (mj ? frr : 8 - R + frr) * 8 + fc]);
       }
     }
-    const T *pu = a.in + (size_t)up * st;
+    const T *pu = in + (size_t)up * st;
     if (more) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) nxt[k] = widen(pu[k * 64 + lane]);
@@ -608,8 +616,16 @@ __global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, CON
       tile[QT::at(k, fjy, R + 8 * mi + fc)] = fj[q];
     }
     __syncthreads();
-    if ((a.mask[(size_t)u * a.kcol + l] >> mm) & 1u) {
-      T *po = a.out + (size_t)b * a.out_step;
+    if ((mk[l] >> mm) & 1u) {
+      T *po = out + (size_t)b * ost;
+      // fused refresh: b's ghost images receive its output too
+      const uint32_t *imgs = nullptr;
+      uint32_t nimg = 0;
+      if (img_off) {
+        const uint32_t i0 = uniform(img_off[b]);
+        nimg = uniform(img_off[b + 1]) - i0;
+        imgs = img + i0;
+      }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         auto X = [&](int kk) -> A {
@@ -626,10 +642,12 @@ __global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, CON
           acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
         }
         if constexpr (COND) {
-          const bool sel = a.cond_mode == 0 ? (acc > A(0)) : (cur[k] > a.theta);
+          const bool sel = cond_mode == 0 ? (acc > A(0)) : (cur[k] > theta);
           acc = sel ? acc : -acc;
         }
         store_nt(&po[k * 64 + lane], acc);
+        for (uint32_t q = 0; q < nimg; ++q)
+          store_nt(&out[(size_t)uniform(imgs[q]) * ost + k * 64 + lane], acc);
       }
     }
 #pragma unroll
@@ -640,6 +658,50 @@ __global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, CON
     }
     b = up;
     __syncthreads();  // every wave's reads of this level's tile are done
+  }
+}
+
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, COND> a) {
+  using A = typename acc_of<T>::type;
+  __shared__ A tile[8 * QuadTile<R, sizeof(A)>::PL];
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;  // uniform over the workgroup
+  A c[6 * R + 1];
+#pragma unroll
+  for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
+  quad_unit<T, R, COND>(a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], (int)a.len[u],
+                        a.mask + (size_t)u * a.kcol, c, a.theta, a.cond_mode, a.img_off, a.img,
+                        tile);
+}
+
+// Batched quad schedule: one persistent launch serves every request of a
+// group table whose bricks come as a quad plan; work item = one quad unit of
+// one request, taken by a whole workgroup.
+template <class T, int R, bool COND>
+__global__ __launch_bounds__(256) void star_sweep_quads_batched(const BatchTable *tab_,
+                                                                uint32_t tick) {
+  using A = typename acc_of<T>::type;
+  __shared__ A tile[8 * QuadTile<R, sizeof(A)>::PL];
+  const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
+  const uint32_t nsegs = tab->nsegs;
+  const uint32_t total = tab->prefix[nsegs];
+  SegCursor cur;
+  for (uint32_t u = xcd_remap(blockIdx.x, gridDim.x); u < total; u += gridDim.x) {
+    cur.seek(tab->prefix, nsegs, u);
+    const auto &sg = tab->seg[cur.s];
+    if (tick >= sg.ticks) continue;  // the same unit for every wave: uniform
+    const uint32_t ul = u - cur.lo;
+    const bool odd = ((sg.parity + tick) & 1u) != 0;
+    const T *in = static_cast<const T *>(odd ? sg.f1 : sg.f0);
+    T *out = static_cast<T *>(odd ? sg.f0 : sg.f1);
+    const kconst_ptr<A> cp = (kconst_ptr<A>)sg.coeff;
+    A c[6 * R + 1];
+#pragma unroll
+    for (int q = 0; q < 6 * R + 1; ++q) c[q] = cp[q];
+    quad_unit<T, R, COND>(sg.adj, in, out, sg.step, sg.step, sg.items[ul], (int)sg.len[ul],
+                          sg.mask + (size_t)ul * sg.kcol, c, (A)sg.theta, sg.cond_mode, nullptr,
+                          nullptr, tile);
   }
 }
 
@@ -772,7 +834,8 @@ hipError_t launch_quads_t(const SweepLaunch &L, const PlanView &P, hipStream_t s
   a.start = P.start;
   a.nunits = P.nunits;
   a.kcol = P.kcol;
-  a.img_off = a.img = nullptr;
+  a.img_off = L.img_off;
+  a.img = L.img;
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
   if (P.nunits == 0) return hipSuccess;
@@ -929,6 +992,34 @@ hipError_t launch_cols_batched_t(const BatchTable *tab, uint32_t tick, hipStream
   return hipGetLastError();
 }
 
+template <class T, int R, bool COND>
+hipError_t launch_quads_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  static int nblk = 0;
+  if (nblk == 0) {
+    int dev = 0, cus = 256, per = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per, star_sweep_quads_batched<T, R, COND>, 256, 0);
+    nblk = cus * (per > 0 ? per : 1);
+  }
+  hipLaunchKernelGGL((star_sweep_quads_batched<T, R, COND>), dim3(nblk), dim3(256), 0, s, tab,
+                     tick);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_quads_batched_dt(int stencil, const BatchTable *tab, uint32_t tick,
+                                   hipStream_t s) {
+  switch (stencil) {
+    case BK_STAR7: return launch_quads_batched_t<T, 1, false>(tab, tick, s);
+    case BK_STAR13: return launch_quads_batched_t<T, 2, false>(tab, tick, s);
+    case BK_STAR25: return launch_quads_batched_t<T, 4, false>(tab, tick, s);
+    case BK_COND7: return launch_quads_batched_t<T, 1, true>(tab, tick, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <class T>
 hipError_t launch_cols_batched_dt(int stencil, const BatchTable *tab, uint32_t tick,
                                   hipStream_t s) {
@@ -1019,6 +1110,16 @@ hipError_t launch_star_columns_batched(int stencil, int dtype, const BatchTable 
     case BK_F64: return launch_cols_batched_dt<double>(stencil, tab, tick, s);
     case BK_F32: return launch_cols_batched_dt<float>(stencil, tab, tick, s);
     case BK_BF16: return launch_cols_batched_dt<bf16>(stencil, tab, tick, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_star_quads_batched(int stencil, int dtype, const BatchTable *tab,
+                                     uint32_t tick, hipStream_t s) {
+  switch (dtype) {
+    case BK_F64: return launch_quads_batched_dt<double>(stencil, tab, tick, s);
+    case BK_F32: return launch_quads_batched_dt<float>(stencil, tab, tick, s);
+    case BK_BF16: return launch_quads_batched_dt<bf16>(stencil, tab, tick, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -101,11 +101,14 @@ class Server:
     batching over `lanes` batches (one stream each), requests admitted
     against an HBM budget, chunks of `chunk` ticks, the next chunk queued
     before the current one is waited for (`pipeline`), and with `tune` each
-    request's schedule chosen by the autotuner per size class."""
+    request's schedule chosen by the autotuner per size class; otherwise
+    stars sweep by `star_plan` ("quad": 2x2 brick blocks, "column": k-columns
+    of 8, None: one brick per wave) and the cube by 2x2 column quads."""
 
     def __init__(self, hbm_budget: float = 200e9, chunk: int = 16, max_active: int = 2048,
                  use_graph: bool = True, ghost: int = 0, slab_bytes: int = 1 << 30,
-                 lanes: int = 2, pipeline: bool = True, tune: bool = False):
+                 lanes: int = 2, pipeline: bool = True, tune: bool = False,
+                 star_plan: str | None = "quad"):
         if lanes not in (1, 2):
             raise ValueError("lanes must be 1 or 2")
         cfg = _lib.bk_serve_config()
@@ -118,6 +121,8 @@ class Server:
         cfg.pipeline = int(bool(pipeline))
         cfg.tune = int(bool(tune))
         cfg.slab_bytes = int(slab_bytes)
+        cfg.star_plan = {None: 0, "quad": _lib.BK_PLAN_QUAD,
+                         "column": _lib.BK_PLAN_COLUMN}[star_plan]
         h = C.c_void_p()
         check(load().bk_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h.value

@@ -378,7 +378,8 @@ int bk_sweeps_deep(const bk_sweep_desc *d, const uint32_t *d_all, uint32_t nall,
  * bk_ghost_images, uploaded), so the next sweep's input is already
  * refreshed.  The same bits as bk_sweeps, with one ghost write per sweep
  * instead of a refresh's read + write.  Star and cond stencils without a
- * plan (one brick per wave); otherwise BK_ERR_UNSUPPORTED. */
+ * plan (one brick per wave) or with a quad plan; otherwise
+ * BK_ERR_UNSUPPORTED. */
 int bk_sweeps_fused(const bk_sweep_desc *d, const uint32_t *d_img_off,
                     const uint32_t *d_img, const uint32_t *d_pairs, uint32_t npairs,
                     int nsweeps, int use_graph, bk_stream s);
@@ -399,8 +400,8 @@ typedef struct bk_tune_cand {
 
 /* flags of bk_tune_sweep: the layouts to try (bit l = enum bk_layout l;
  * at least one), and a restriction to the schedules the batched kernels
- * run (8x8x8 only: per brick with 4 bricks per workgroup or k-columns for
- * the stars, 2x2 column quads for the cube). */
+ * run (8x8x8 only: per brick with 4 bricks per workgroup, k-columns or
+ * 2x2 brick blocks for the stars, 2x2 column quads for the cube). */
 #define BK_TUNE_LAYOUT_8x8x8 0x1u
 #define BK_TUNE_LAYOUT_4x8x16 0x2u
 #define BK_TUNE_LAYOUT_4x8x16_F88 0x4u
@@ -415,8 +416,8 @@ typedef struct bk_tune_cand {
  * builds the grid in every selected layout the extents fit, runs every
  * schedule of that layout -- per brick with 1, 2, 4 or 8 bricks per
  * workgroup (8: 8x8x8 stars only; 4: the cube on 8x8x8), k-columns of
- * 4/8/16 bricks and 2x2 column quads of 4/8 levels (8x8x8 stars), quads of
- * 2/4/8 levels (8x8x8 cube) -- untimed for warm_s seconds, then times `steps` ping-pong sweeps
+ * 4/8/16 bricks and 2x2 column quads of 1/2/4/8 levels (8x8x8 stars),
+ * quads of 2/4/8 levels (8x8x8 cube) -- untimed for warm_s seconds, then times `steps` ping-pong sweeps
  * per candidate (hipGraph replays) in `rounds` interleaved rounds, keeping
  * each candidate's best.  h_out receives up to max_out candidates fastest
  * first (h_out[0] is the choice); with h_out NULL *nout is the candidate
@@ -437,10 +438,10 @@ int bk_tune_lookup(int stencil, int dtype, const long cells[3], int ghost,
  * every request with t < ticks refreshes the ghost bricks of its input
  * field (pairs from bk_ghost_pairs, may be none) and sweeps it into the
  * other field: the input is f0 when (parity + t) is even, f1 otherwise.
- * Stars sweep the bricks of d_list, or, when `plan` is set (a
- * BK_PLAN_COLUMN plan), the bricks of that plan by the k-marching column
- * schedule; the cube sweeps the bricks of `plan`, which must be a
- * BK_PLAN_QUAD plan.  Either way the results are those of bk_sweep.  This is the weak-scaling loop of
+ * Stars sweep the bricks of d_list, or, when `plan` is set, the bricks of
+ * that plan by the k-marching column schedule (BK_PLAN_COLUMN) or in 2x2
+ * brick blocks (BK_PLAN_QUAD); the cube sweeps the bricks of `plan`, which
+ * must be a BK_PLAN_QUAD plan.  Either way the results are those of bk_sweep.  This is the weak-scaling loop of
  * weak/main.hip.cpp:243-284 for many subdomains at once, the way the
  * strong-scaling driver's one launch over many subdomains does
  * (strong/main.cu:85-99), with grids of any size, stencil and dtype. */
@@ -450,7 +451,7 @@ typedef struct bk_batch_req {
   const uint32_t *d_adj;
   const uint32_t *d_list; /* stars without a plan: bricks to compute */
   uint32_t nlist;
-  bk_plan plan;           /* cube: BK_PLAN_QUAD plan; stars: BK_PLAN_COLUMN or NULL */
+  bk_plan plan;           /* cube: BK_PLAN_QUAD plan; stars: a column or quad plan, or NULL */
   const uint32_t *d_pairs; /* ghost (dst, src) pairs, or NULL */
   uint32_t npairs;
   void *d_f0;
@@ -501,6 +502,10 @@ typedef struct bk_serve_config {
   int tune;             /* schedules from bk_tune_sweep per (size class,
                            stencil, dtype); 0: fixed defaults */
   size_t slab_bytes;    /* arena slab size (0: 1 GiB) */
+  int star_plan;        /* fixed default for stars and cond: 0 one brick
+                           per wave, BK_PLAN_QUAD 2x2 brick blocks (levels
+                           per stencil and dtype), BK_PLAN_COLUMN k-columns
+                           of 8 */
 } bk_serve_config;
 
 typedef struct bk_serve_req {

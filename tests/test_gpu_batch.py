@@ -116,20 +116,23 @@ def test_poisson_stream_small(gpu):
 
 @pytest.mark.parametrize("ghost", [0, 1])
 def test_star_schedules_in_one_batch(gpu, ghost):
-    """Star requests by both batched schedules in one batch -- one brick per
-    wave and k-marching column units (a BK_PLAN_COLUMN plan in the request)
+    """Star requests by all three batched schedules in one batch -- one
+    brick per wave, k-marching column units (a BK_PLAN_COLUMN plan in the
+    request) and 2x2 brick blocks (a BK_PLAN_QUAD plan, one to three levels)
     -- each give the single-grid result bit for bit, for every star."""
     shapes = ShapeCache()
     reqs, q = [], 0
     for name in ("7pt", "13pt", "25pt", "cond"):
         for dt in (BK_F64, BK_F32, BK_BF16):
-            for columns in (False, True):
+            for sched in ("brick", "columns", "quad1", "quad3"):
                 nb = ((2, 3, 4), (3, 1, 5), (1, 2, 2), (4, 2, 3))[q % 4]
                 r = Request(rid=q, nb=nb, stencil=Stencil.make(name, seed=200 + q), dtype=dt,
                             steps=3 + q % 2, seed=300 + q, ghost=ghost)
                 r.allocate(shapes)
                 rq = r.req_struct(0)
-                rq.plan = r.shape.columns.handle if columns else None
+                rq.plan = {"brick": None, "columns": r.shape.columns.handle,
+                           "quad1": r.shape.grid.plan("quad", 1).handle,
+                           "quad3": r.shape.grid.plan("quad", 3).handle}[sched]
                 r._rec = bytes(rq)
                 reqs.append(r)
                 q += 1
@@ -159,17 +162,20 @@ def test_malformed_requests_rejected(gpu):
     rq.d_f1 = rq.d_f0  # aliased fields
     arr = (bk_batch_req * 1)(rq)
     assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 1
-    # a star request with a quad plan (stars take column plans only)
+    # a star request takes a column or a quad plan
     s = Request(rid=1, nb=(2, 2, 2), stencil=Stencil.make("13pt"), dtype=BK_F64, steps=1,
                 seed=2)
     s.allocate(shapes)
     rq = s.req_struct(1)
-    rq.plan = s.shape.quad.handle
-    arr = (bk_batch_req * 1)(rq)
-    assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 1
+    for pl in (s.shape.quad, s.shape.columns):
+        rq.plan = pl.handle
+        arr = (bk_batch_req * 1)(rq)
+        assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 0
+    # a cube request needs a quad plan, not a column plan
+    rq = r.req_struct(1)
     rq.plan = s.shape.columns.handle
     arr = (bk_batch_req * 1)(rq)
-    assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 0
+    assert load().bk_batch_set(C.c_void_p(b.handle), arr, 1, None) == 1
     with pytest.raises(BrickError):
         Batch(0)
     with pytest.raises(ValueError):

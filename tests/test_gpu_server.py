@@ -88,7 +88,7 @@ def test_tuned_schedules_give_the_same_bits(gpu, ghost):
         if r.stencil.kind == 3:
             assert r.schedule[0] == _lib.BK_PLAN_QUAD and r.schedule[1] in (2, 4, 8)
         else:
-            assert r.schedule[0] in (0, _lib.BK_PLAN_COLUMN)
+            assert r.schedule[0] in (0, _lib.BK_PLAN_COLUMN, _lib.BK_PLAN_QUAD)
         print(f"[case] tuned rid={r.rid} schedule={r.schedule} ghost={ghost}", flush=True)
         np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
 
@@ -113,3 +113,19 @@ def test_server_rejects_malformed(gpu):
     assert lib.bk_server_submit(C.c_void_p(srv.handle), C.byref(q), C.byref(sid)) != 0
     with pytest.raises(BrickError):
         serve.Server(lanes=2, chunk=0)
+
+
+@pytest.mark.parametrize("star_plan", [None, "column", "quad"])
+@pytest.mark.parametrize("ghost", [0, 1])
+def test_star_plan_choices_give_the_same_bits(gpu, star_plan, ghost):
+    """Every fixed star schedule of the serving loop (one brick per wave,
+    k-columns of 8, 2x2 brick blocks) gives the single-grid results."""
+    srv = serve.Server(chunk=2, ghost=ghost, star_plan=star_plan)
+    reqs = make_mix(base=100)
+    stats = srv.serve(reqs, realtime=False, keep_results=True)
+    assert len(stats.completed) == len(reqs)
+    for r in reqs:
+        want = {None: 0, "column": _lib.BK_PLAN_COLUMN, "quad": _lib.BK_PLAN_QUAD}[star_plan]
+        assert r.schedule[0] == (_lib.BK_PLAN_QUAD if r.stencil.kind == 3 else want)
+        print(f"[case] star_plan={star_plan} ghost={ghost} rid={r.rid}", flush=True)
+        np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
