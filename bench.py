@@ -21,7 +21,9 @@ The same JSON line also carries:
                   13/25/125-point and cond loops) with OpenMP on the host
                   cores, per stencil, with the CPU model and core counts;
   by_stencil   -- 13/25-point, 125-point cube and cond at the same size,
-                  plus the FP32 / BF16 forms of the 7-point star;
+                  the FP32 / BF16 forms of the 7-point star, FP32 25-point
+                  and cube, and the 7/25-point block kernels with the
+                  i-direction taps from DPP lane shifts (*_lane_shift);
   precision    -- FP32 and BF16-storage forms against the FP64 sweep after
                   1, 10 and 100 sweeps at 512^3 and 1024^3 (errors computed
                   in FP64 on the GPU) and their speed-up over FP64 per step
@@ -456,12 +458,17 @@ def main():
         vpeak = valu_peak(bk, stream)
         print(f"[bench] measured VALU peak: {vpeak}", file=sys.stderr, flush=True)
         only = set(x for x in args.only.split(",") if x)
-        cases = [("13pt_f64", "13pt", bk.BK_F64), ("25pt_f64", "25pt", bk.BK_F64),
-                 ("125pt_f64", "125pt", bk.BK_F64), ("cond_f64", "cond", bk.BK_F64),
-                 ("7pt_f32", "7pt", bk.BK_F32), ("7pt_bf16", "7pt", bk.BK_BF16),
-                 ("125pt_f32", "125pt", bk.BK_F32)]
+        # the *_lane_shift cases: the same default schedule (2x2 brick
+        # blocks) with the i-direction taps from DPP lane shifts
+        cases = [("13pt_f64", "13pt", bk.BK_F64, {}), ("25pt_f64", "25pt", bk.BK_F64, {}),
+                 ("125pt_f64", "125pt", bk.BK_F64, {}), ("cond_f64", "cond", bk.BK_F64, {}),
+                 ("7pt_f64_lane_shift", "7pt", bk.BK_F64, {"lane_shift": 1}),
+                 ("25pt_f64_lane_shift", "25pt", bk.BK_F64, {"lane_shift": 1}),
+                 ("7pt_f32", "7pt", bk.BK_F32, {}), ("25pt_f32", "25pt", bk.BK_F32, {}),
+                 ("25pt_f32_lane_shift", "25pt", bk.BK_F32, {"lane_shift": 1}),
+                 ("7pt_bf16", "7pt", bk.BK_BF16, {}), ("125pt_f32", "125pt", bk.BK_F32, {})]
         cur = {bk.BK_F64: (a, b)}
-        for name, sname, dt in cases:
+        for name, sname, dt, kw in cases:
             if only and name not in only:
                 continue
             if dt not in cur:
@@ -469,9 +476,9 @@ def main():
             fa, fb = cur[dt]
             st = S.Stencil.make(sname, seed=1)
             steps = max(10, args.steps // (5 if sname == "125pt" else 1))
-            ms_s = time_sweeps(bk, st, fa, fb, g, stream, steps, 2, warm_s=args.warm_s)
+            ms_s = time_sweeps(bk, st, fa, fb, g, stream, steps, 2, warm_s=args.warm_s, **kw)
             ms_k = time_sweeps(bk, st, fa, fb, g, stream, steps, 1, refresh=False,
-                               warm_s=args.warm_s / 2)
+                               warm_s=args.warm_s / 2, **kw)
             es = {bk.BK_F64: 8, bk.BK_F32: 4, bk.BK_BF16: 2}[dt]
             fl = S.flops_per_point(st.kind)
             gb = 2 * es * pts / (ms_k * 1e-3) / 1e9

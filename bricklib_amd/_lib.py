@@ -9,6 +9,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbricks.so")
+API_VERSION = 3  # BK_API_VERSION of include/bk_api.h
 
 BK_OK = 0
 BK_ERR_ARG = 1
@@ -48,6 +49,7 @@ class bk_sweep_desc(C.Structure):
         ("plan", vp),
         ("layout", C.c_int),
         ("wg_bricks", C.c_int),
+        ("lane_shift", C.c_int),
     ]
 
 
@@ -117,6 +119,7 @@ class bk_tune_cand(C.Structure):
         ("plan_shape", C.c_int),
         ("kcol", C.c_uint32),
         ("wg_bricks", C.c_int),
+        ("lane_shift", C.c_int),
         ("ms", C.c_float),
     ]
 
@@ -276,6 +279,11 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    # the structures above follow one version of include/bk_api.h
+    if lib.bk_version() != API_VERSION:
+        raise ImportError(
+            f"{LIB_PATH} implements C-ABI version {lib.bk_version()}, this binding expects "
+            f"{API_VERSION}; rebuild with `python -m bricklib_amd.build`")
     _lib = lib
     return lib
 

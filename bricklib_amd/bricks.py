@@ -356,6 +356,9 @@ DEFAULT_PLANS = {
     (1, BK_BF16): ("quad", 2), (4, BK_BF16): ("quad", 2), (2, BK_BF16): ("quad", 4),
     (3, BK_F64): ("quad", 4), (3, BK_F32): ("quad", 4), (3, BK_BF16): ("quad", 4),
 }
+# lane_shift of the quad star kernels per (stencil, dtype); absent = 0 (the
+# i-taps from the LDS tile).
+DEFAULT_LANE_SHIFT: dict = {}
 
 
 class PeriodicGrid:
@@ -484,8 +487,9 @@ class PeriodicGrid:
         if choice is None:
             return None
         if int(stencil_kind) == _lib.BK_CUBE125 and self.interior.size > (1 << 20):
-            # large grids: 8 levels (1024^3 FP64: 4.76 ms per sweep, the same
-            # per cell as 512^3 with 4 levels, against 5.0-6.2 ms with 4)
+            # large grids: 8 levels (1024^3 FP64: 4.76 ms per sweep on one
+            # host against 5.0-6.2 with 4; on hosts whose clock drops under
+            # the long FP64 kernel every schedule lands at 5.8-6.1 ms)
             choice = ("quad", 8)
         return self.plan(*choice)
 
@@ -494,6 +498,14 @@ class PeriodicGrid:
         else the kernels' default (0)."""
         tuned = self._tuned.get(int(stencil_kind))
         return tuned.wg_bricks if tuned is not None else 0
+
+    def auto_lane_shift(self, stencil_kind: int, dtype: int = BK_F64) -> int:
+        """lane_shift of the quad star kernels: the tuned value, else
+        DEFAULT_LANE_SHIFT."""
+        tuned = self._tuned.get(int(stencil_kind))
+        if tuned is not None:
+            return int(tuned.lane_shift)
+        return DEFAULT_LANE_SHIFT.get((int(stencil_kind), int(dtype)), 0)
 
     @property
     def cell_extents(self) -> tuple:
@@ -579,7 +591,7 @@ def _resolve_plan(plan, st: Stencil, grid: PeriodicGrid, dtype: int = BK_F64):
 
 def _desc(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, coeff_keep: list,
           blist: DeviceBuffer | None = None, nlist: int | None = None,
-          plan=None, wg_bricks=None) -> bk_sweep_desc:
+          plan=None, wg_bricks=None, lane_shift=None) -> bk_sweep_desc:
     if out.dtype != inp.dtype:
         raise ValueError("input and output fields must share a dtype")
     coeff = np.ascontiguousarray(st.coeff, dtype=np.float64)
@@ -608,23 +620,31 @@ def _desc(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, coeff_keep: l
         wg_bricks = (grid.auto_wg_bricks(st.kind)
                      if plan == "auto" and hasattr(grid, "auto_wg_bricks") else 0)
     d.wg_bricks = 0 if pl is not None else int(wg_bricks)
+    if lane_shift is None:
+        # automatic: only where the kernels have it (quad plans of the stars)
+        quad_star = pl is not None and pl.shape == "quad" and st.kind != _lib.BK_CUBE125
+        lane_shift = (grid.auto_lane_shift(st.kind, inp.dtype)
+                      if quad_star and hasattr(grid, "auto_lane_shift") else 0)
+    d.lane_shift = int(lane_shift)
     return d
 
 
 def sweep(st: Stencil, out: Brick, inp: Brick, grid: PeriodicGrid, stream=None,
-          plan="auto", wg_bricks=None) -> None:
+          plan="auto", wg_bricks=None, lane_shift=None) -> None:
     """One sweep over the interior bricks: out = stencil(in).  ``plan`` is
     "auto", None (plain brick list, one brick per wave), "column", "quad"
     or a Plan; ``wg_bricks`` the bricks per workgroup of the per-brick
-    kernels (None: tuned or default)."""
+    kernels (None: tuned or default); ``lane_shift`` 1 for the i-taps by
+    DPP lane shifts in the quad star kernels (None: tuned or default)."""
     keep: list = []
-    d = _desc(st, out, inp, grid, keep, plan=plan, wg_bricks=wg_bricks)
+    d = _desc(st, out, inp, grid, keep, plan=plan, wg_bricks=wg_bricks, lane_shift=lane_shift)
     check(load().bk_sweep(C.byref(d), _vp(_stream_of(stream))))
 
 
 def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
            use_graph: bool = False, refresh: bool = True, stream=None, plan="auto",
-           ghost_iters: int = 1, fused: bool = False, wg_bricks=None) -> Brick:
+           ghost_iters: int = 1, fused: bool = False, wg_bricks=None,
+           lane_shift=None) -> Brick:
     """``nsweeps`` ping-pong sweeps a->b->a..., each preceded by the periodic
     ghost refresh of its input -- or, with ghost_iters > 1, the refresh once
     per ghost_iters sweeps and the sweeps in between over every brick of the
@@ -633,7 +653,9 @@ def sweeps(st: Stencil, a: Brick, b: Brick, grid: PeriodicGrid, nsweeps: int,
     wave or by quad plans).  Same result in every form.  Returns the field
     holding the result."""
     keep: list = []
-    d = _desc(st, b, a, grid, keep, plan=plan, wg_bricks=wg_bricks)
+    if ghost_iters != 1 and lane_shift is None:
+        lane_shift = 0  # the sweeps over every brick run without quad plans
+    d = _desc(st, b, a, grid, keep, plan=plan, wg_bricks=wg_bricks, lane_shift=lane_shift)
     pairs = grid.d_pairs.ptr if (refresh and grid.d_pairs is not None) else None
     npairs = grid.npairs if refresh else 0
     if fused:
@@ -667,7 +689,7 @@ def cand_name(c) -> str:
     if c.plan_shape == 0:
         return f"{lay} brick wg{c.wg_bricks}"
     shape = "quad" if c.plan_shape == _lib.BK_PLAN_QUAD else "column"
-    return f"{lay} {shape}{c.kcol}"
+    return f"{lay} {shape}{c.kcol}" + ("+dpp" if c.lane_shift else "")
 
 
 def tune(st: Stencil, cells, dtype: int = BK_F64, ghost: int = 1, steps: int = 5,
@@ -675,7 +697,8 @@ def tune(st: Stencil, cells, dtype: int = BK_F64, ghost: int = 1, steps: int = 5
          stream=None) -> list:
     """bk_tune_sweep: time every (layout, schedule) candidate for stencil
     `st` on a periodic grid of `cells` (i, j, k) and return them fastest
-    first (bk_tune_cand records: layout, plan_shape, kcol, wg_bricks, ms)."""
+    first (bk_tune_cand records: layout, plan_shape, kcol, wg_bricks,
+    lane_shift, ms)."""
     flags = 0
     for lay in layouts:
         flags |= 1 << int(lay)

@@ -30,6 +30,7 @@ struct SweepLaunch {
   const uint32_t *img = nullptr;
   int layout = BK_LAYOUT_8x8x8;  // brick shape and fold of both fields
   int wg_bricks = 0;             // bricks per workgroup (0: the kernel's default)
+  int lane_shift = 0;            // quad star kernels: i-taps by DPP lane shifts
 };
 
 // Device view of an execution plan (see bk_plan.hip).

@@ -167,6 +167,12 @@ int check_desc(const bk_sweep_desc *d, bool pingpong = false) {
                 "stars only");
   if (d->layout != BK_LAYOUT_8x8x8 && d->plan)
     return fail(BK_ERR_UNSUPPORTED, "bk_sweep: plans are built for the 8x8x8 layout only");
+  if (d->lane_shift != 0 && d->lane_shift != 1)
+    return fail(BK_ERR_ARG, "bk_sweep: lane_shift must be 0 or 1 (got %d)", d->lane_shift);
+  if (d->lane_shift &&
+      (d->stencil == BK_CUBE125 || !d->plan || bk::plan_view(d->plan).shape != BK_PLAN_QUAD))
+    return fail(BK_ERR_UNSUPPORTED, "bk_sweep: lane_shift is built into the star and cond "
+                "kernels of quad plans only");
   if (!d->plan && d->nlist == 0) return BK_OK;
   if (!d->d_adj || !d->d_in || !d->d_out || (!d->plan && !d->d_list))
     return fail(BK_ERR_ARG, "bk_sweep: null device pointer");
@@ -210,6 +216,7 @@ void to_launch(const bk_sweep_desc *d, bk::SweepLaunch &L) {
   L.cond_mode = d->cond_mode;
   L.layout = d->layout;
   L.wg_bricks = d->wg_bricks;
+  L.lane_shift = d->lane_shift;
 }
 
 hipError_t launch_sweep(const bk::SweepLaunch &L, const bk_plan_s *plan, hipStream_t s) {
@@ -711,6 +718,7 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
   put(&d->cond_mode, sizeof(d->cond_mode));
   put(&d->layout, sizeof(d->layout));
   put(&d->wg_bricks, sizeof(d->wg_bricks));
+  put(&d->lane_shift, sizeof(d->lane_shift));
   put(&d_pairs, sizeof(d_pairs));
   put(&npairs, sizeof(npairs));
   put(&nsweeps, sizeof(nsweeps));

@@ -58,6 +58,50 @@ __device__ __forceinline__ A tapv(A x) {
   else return x;
 }
 
+// Lane shifts inside the 16-lane DPP rows (two i-rows of the 8x8 plane
+// fold): row_shl:n gives lane l the value of lane l + n, row_shr:n that of
+// lane l - n, for 1 <= n <= 7 (lanes whose source leaves the row read 0 and
+// are never used below).  64-bit values move as two 32-bit halves.
+template <bool UP>
+__device__ __forceinline__ uint32_t row_shift32(uint32_t v, int n) {
+  // n is a constant after unrolling: the switch folds to one DPP move
+  switch (n) {
+#define BK_ROWSH(N)                                                              \
+  case N:                                                                        \
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, UP ? 0x100 + N : 0x110 + N, \
+                                              0xf, 0xf, true);
+    BK_ROWSH(1) BK_ROWSH(2) BK_ROWSH(3) BK_ROWSH(4) BK_ROWSH(5) BK_ROWSH(6) BK_ROWSH(7)
+#undef BK_ROWSH
+    default: return 0u;
+  }
+}
+
+template <bool UP>
+__device__ __forceinline__ float row_shift(float x, int n) {
+  return __builtin_bit_cast(float, row_shift32<UP>(__builtin_bit_cast(uint32_t, x), n));
+}
+
+template <bool UP>
+__device__ __forceinline__ double row_shift(double x, int n) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = row_shift32<UP>((uint32_t)u, n), hi = row_shift32<UP>((uint32_t)(u >> 32), n);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// The i + r (UP) or i - r tap of lane (j, i) of an 8x8 plane x, 1 <= r <= 7,
+// from registers: inside the brick a shift of x by r lanes; past its edge
+// the neighbour brick's cell, taken from nb = that brick's cells at the
+// same (j, i) (the i + 8 or i - 8 cells), shifted by 8 - r the other way.
+// Both shifts run with every lane active and a select picks one: a DPP move
+// reads a source lane that EXEC disables as 0, so neither may sit in a
+// branch on i.
+template <bool UP, class A>
+__device__ __forceinline__ A itap_shift(A x, A nb, int i, int r) {
+  const A in_brick = row_shift<UP>(x, r);
+  const A past_edge = row_shift<!UP>(nb, 8 - r);
+  return (UP ? i + r < 8 : i - r >= 0) ? in_brick : past_edge;
+}
+
 // LDS tile of one brick and its R-deep i/j faces: planes k of P x P cells
 // (P = 8 + 2R) of ES bytes, element (k, y, x) at at(k, y, x).  For R = 4
 // (16-wide rows) bit 3 of x is flipped so that the rows one read pass
@@ -536,12 +580,57 @@ struct QuadTile {
   __device__ static constexpr int at(int k, int y, int x) { return k * PL + y * RS + x; }
 };
 
+// Tile planes staged per pass of a quad level: 8 (one pass), or 4 (two
+// passes, half the LDS per workgroup, so twice the workgroups per CU where
+// LDS is what limits them).  Bit R of BK_QUAD_HALF_R64 (8-byte cells) or
+// BK_QUAD_HALF_R32 (4-byte cells) selects two passes for radius R.  Measured
+// at 512^3 (kernel ms, one pass / two): FP32 25-point 4 levels 0.250 / 0.239,
+// 2 levels 0.264 / 0.241; FP64 25-point 0.468 / 0.482 (the extra barriers,
+// no more waves: registers hold it at 4 per SIMD either way), FP64 13-point
+// 0.412 / 0.433, FP32 13-point 0.210 / 0.216.
+#ifndef BK_QUAD_HALF_R64
+#define BK_QUAD_HALF_R64 0x0
+#endif
+#ifndef BK_QUAD_HALF_R32
+#define BK_QUAD_HALF_R32 0x10
+#endif
+template <class A, int R>
+constexpr int quad_tile_planes() {
+  return (((sizeof(A) == 8 ? BK_QUAD_HALF_R64 : BK_QUAD_HALF_R32) >> R) & 1) ? 4 : 8;
+}
+
+// The two outer faces one level ahead: a unit's first faces are issued
+// before its brick planes, and each next level's faces during this level's
+// compute (2R more values per lane) instead of at the start of the next
+// level, where the tile fill waits for them.  Bit R of BK_QUAD_PF_R64 /
+// BK_QUAD_PF_R32.  Measured at 512^3 (kernel ms, at the start / ahead):
+// FP64 7-point 1 level 0.397 / 0.392, cond 0.398 / 0.392, 25-point 2 levels
+// 0.465 / 0.458 (with 4 waves per SIMD kept, below), FP32 7-point 0.206 /
+// 0.202, 25-point 2 levels 0.238 / 0.227.
+#ifndef BK_QUAD_PF_R64
+#define BK_QUAD_PF_R64 0x16
+#endif
+#ifndef BK_QUAD_PF_R32
+#define BK_QUAD_PF_R32 0x16
+#endif
+template <class A, int R>
+constexpr bool quad_face_prefetch() {
+  return (((sizeof(A) == 8 ? BK_QUAD_PF_R64 : BK_QUAD_PF_R32) >> R) & 1) != 0;
+}
+
 // One quad unit: the four waves of a workgroup (member mm = wave index)
 // march L levels from the unit's anchor; mk[l] holds level l's store mask
 // (bit mm: store member mm).  Every wave of the workgroup must call it with
-// the same unit (it synchronises the workgroup once per level).  Shared by
-// the single-grid and the batched kernels.
-template <class T, int R, bool COND>
+// the same unit (it synchronises the workgroup twice per pass of a level
+// through the tile).  Shared by
+// the single-grid and the batched kernels.  SHIFT: the i-direction taps come
+// from DPP lane shifts of the wave's own plane and of the two cells eight
+// columns away (one tile read each per plane) instead of 2R tile reads; the
+// same values, so the same bits.
+// PF: the outer faces one level ahead (quad_face_prefetch; the
+// single-grid kernel -- the persistent batched kernel keeps those registers
+// for its occupancy).
+template <class T, int R, bool COND, bool SHIFT = false, bool PF = false>
 __device__ __forceinline__ void quad_unit(const uint32_t *__restrict__ adj,
                                           const T *__restrict__ in, T *__restrict__ out,
                                           size_t st, size_t ost, uint32_t anchor, int L,
@@ -553,6 +642,7 @@ __device__ __forceinline__ void quad_unit(const uint32_t *__restrict__ adj,
                                           typename acc_of<T>::type *tile) {
   using A = typename acc_of<T>::type;
   using QT = QuadTile<R, sizeof(A)>;
+  constexpr int KH = quad_tile_planes<A, R>(), NH = 8 / KH;
   const int mm = (int)uniform(threadIdx.x >> 6);  // member: i-offset mm & 1, j-offset mm >> 1
   const int lane = threadIdx.x & 63;
   const int j = lane >> 3, i = lane & 7;
@@ -572,7 +662,21 @@ __device__ __forceinline__ void quad_unit(const uint32_t *__restrict__ adj,
   const int fix = mi ? R + 16 + frr : frr;  // the i-face's tile column
   const int fjy = mj ? R + 16 + frr : frr;  // the j-face's tile row
 
-  A cur[8], nxt[8], km[R];
+  // the two outer faces of brick bb (its i- and j-neighbours' R-deep layers)
+  auto load_faces = [&](uint32_t bb, A(&ffi)[R], A(&ffj)[R]) {
+    const uint32_t *abb = adj + (size_t)bb * 27;
+    const T *pi = in + (size_t)uniform(abb[si]) * st;
+    const T *pj = in + (size_t)uniform(abb[sj]) * st;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = q * (8 / R) + fkq;
+      ffi[q] = widen(pi[k * 64 + fc * 8 + (mi ? frr : 8 - R + frr)]);
+      ffj[q] = widen(pj[k * 64 + (mj ? frr : 8 - R + frr) * 8 + fc]);
+    }
+  };
+
+  A cur[8], nxt[8], km[R], fi[R], fj[R];
+  if constexpr (PF) load_faces(b, fi, fj);
   {
     const T *p = in + (size_t)b * st;
 #pragma unroll
@@ -585,20 +689,10 @@ __device__ __forceinline__ void quad_unit(const uint32_t *__restrict__ adj,
     const uint32_t *ab = adj + (size_t)b * 27;
     const uint32_t up = uniform(ab[22]);
     const bool more = l + 1 < L;
-    // the two outer faces first (waited for before the tile is filled),
-    // then the prefetch of the next level, which stays in flight through
-    // the compute below
-    A fi[R], fj[R];
-    {
-      const T *pi = in + (size_t)uniform(ab[si]) * st;
-      const T *pj = in + (size_t)uniform(ab[sj]) * st;
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int k = q * (8 / R) + fkq;
-        fi[q] = widen(pi[k * 64 + fc * 8 + (mi ? frr : 8 - R + frr)]);
-        fj[q] = widen(pj[k * 64 + (mj ? frr : 8 - R + frr) * 8 + fc]);
-      }
-    }
+    // the two outer faces first (waited for before the tile is filled;
+    // with PF loaded one level ahead), then the prefetch of the next level,
+    // which stays in flight through the compute below
+    if constexpr (!PF) load_faces(b, fi, fj);
     const T *pu = in + (size_t)up * st;
     if (more) {
 #pragma unroll
@@ -607,82 +701,132 @@ __device__ __forceinline__ void quad_unit(const uint32_t *__restrict__ adj,
 #pragma unroll
       for (int r = 0; r < R; ++r) nxt[r] = widen(pu[r * 64 + lane]);
     }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) tile[QT::at(k, ty, tx)] = cur[k];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = q * (8 / R) + fkq;
-      tile[QT::at(k, R + 8 * mj + fc, fix)] = fi[q];
-      tile[QT::at(k, fjy, R + 8 * mi + fc)] = fj[q];
+    A fni[R], fnj[R];
+    if constexpr (PF) {
+      if (more) load_faces(up, fni, fnj);
     }
-    __syncthreads();
-    if ((mk[l] >> mm) & 1u) {
-      T *po = out + (size_t)b * ost;
-      // fused refresh: b's ghost images receive its output too
-      const uint32_t *imgs = nullptr;
-      uint32_t nimg = 0;
-      if (img_off) {
-        const uint32_t i0 = uniform(img_off[b]);
-        nimg = uniform(img_off[b + 1]) - i0;
-        imgs = img + i0;
-      }
+    const bool store_me = ((mk[l] >> mm) & 1u) != 0;
+    T *po = out + (size_t)b * ost;
+    // fused refresh: b's ghost images receive its output too
+    const uint32_t *imgs = nullptr;
+    uint32_t nimg = 0;
+    if (img_off && store_me) {
+      const uint32_t i0 = uniform(img_off[b]);
+      nimg = uniform(img_off[b + 1]) - i0;
+      imgs = img + i0;
+    }
+    // the level's planes in NH passes of KH planes through the tile
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        auto X = [&](int kk) -> A {
-          return kk < 0 ? km[kk + R] : (kk >= 8 ? nxt[kk - 8] : cur[kk]);
-        };
-        A acc = c[0] * tapv<A, COND>(cur[k]);
+    for (int h = 0; h < NH; ++h) {
 #pragma unroll
-        for (int r = 1; r <= R; ++r) {
-          acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[QT::at(k, ty, tx + r)]), acc);
-          acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[QT::at(k, ty, tx - r)]), acc);
-          acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[QT::at(k, ty + r, tx)]), acc);
-          acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[QT::at(k, ty - r, tx)]), acc);
-          acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
-          acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
-        }
-        if constexpr (COND) {
-          const bool sel = cond_mode == 0 ? (acc > A(0)) : (cur[k] > theta);
-          acc = sel ? acc : -acc;
-        }
-        store_nt(&po[k * 64 + lane], acc);
-        for (uint32_t q = 0; q < nimg; ++q)
-          store_nt(&out[(size_t)uniform(imgs[q]) * ost + k * 64 + lane], acc);
+      for (int kk = 0; kk < KH; ++kk) tile[QT::at(kk, ty, tx)] = cur[h * KH + kk];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int k = q * (8 / R) + fkq;  // for R = 1 the plane, and so the pass, varies by lane
+        if (k / KH != h) continue;
+        tile[QT::at(k - h * KH, R + 8 * mj + fc, fix)] = fi[q];
+        tile[QT::at(k - h * KH, fjy, R + 8 * mi + fc)] = fj[q];
       }
+      __syncthreads();
+      if (store_me) {
+#pragma unroll
+        for (int kk = 0; kk < KH; ++kk) {
+          const int k = h * KH + kk;
+          auto X = [&](int z) -> A {
+            return z < 0 ? km[z + R] : (z >= 8 ? nxt[z - 8] : cur[z]);
+          };
+          A acc = c[0] * tapv<A, COND>(cur[k]);
+          // SHIFT: the cells 8 columns right and left (the i-neighbour
+          // bricks' same lane; only lanes i < R of the first and i >= 8 - R
+          // of the second are used, the others clamped into the tile's row)
+          A east = A(0), west = A(0);
+          if constexpr (SHIFT) {
+            east = tile[QT::at(kk, ty, tx + 8 < QT::P ? tx + 8 : QT::P - 1)];
+            west = tile[QT::at(kk, ty, tx >= 8 ? tx - 8 : 0)];
+          }
+#pragma unroll
+          for (int r = 1; r <= R; ++r) {
+            if constexpr (SHIFT) {
+              acc = fmadd(c[6 * r - 5], tapv<A, COND>(itap_shift<true>(cur[k], east, i, r)), acc);
+              acc = fmadd(c[6 * r - 4], tapv<A, COND>(itap_shift<false>(cur[k], west, i, r)), acc);
+            } else {
+              acc = fmadd(c[6 * r - 5], tapv<A, COND>(tile[QT::at(kk, ty, tx + r)]), acc);
+              acc = fmadd(c[6 * r - 4], tapv<A, COND>(tile[QT::at(kk, ty, tx - r)]), acc);
+            }
+            acc = fmadd(c[6 * r - 3], tapv<A, COND>(tile[QT::at(kk, ty + r, tx)]), acc);
+            acc = fmadd(c[6 * r - 2], tapv<A, COND>(tile[QT::at(kk, ty - r, tx)]), acc);
+            acc = fmadd(c[6 * r - 1], tapv<A, COND>(X(k + r)), acc);
+            acc = fmadd(c[6 * r], tapv<A, COND>(X(k - r)), acc);
+          }
+          if constexpr (COND) {
+            const bool sel = cond_mode == 0 ? (acc > A(0)) : (cur[k] > theta);
+            acc = sel ? acc : -acc;
+          }
+          store_nt(&po[k * 64 + lane], acc);
+          for (uint32_t q = 0; q < nimg; ++q)
+            store_nt(&out[(size_t)uniform(imgs[q]) * ost + k * 64 + lane], acc);
+        }
+      }
+      __syncthreads();  // every wave's reads of this pass's tile are done
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) km[r] = cur[8 - R + r];
     if (more) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
+      if constexpr (PF) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          fi[q] = fni[q];
+          fj[q] = fnj[q];
+        }
+      }
     }
     b = up;
-    __syncthreads();  // every wave's reads of this level's tile are done
   }
 }
 
-template <class T, int R, bool COND>
-__global__ __launch_bounds__(256) void star_sweep_quads(const StarArgs<T, R, COND> a) {
+// Waves per SIMD the compiler must keep room for (its register budget is
+// 512 / N VGPRs per lane); 1 leaves the choice to it.  Radius 4 in FP64 with
+// the face prefetch would otherwise take 152 VGPRs (3 waves per SIMD: 25-point
+// 0.502 ms against 0.458 at 4 waves in 110 VGPRs); 5 or 6 waves spill
+// (0.54-0.79 ms).
+#ifndef BK_QUAD_MINW_R4_64
+#define BK_QUAD_MINW_R4_64 4
+#endif
+#ifndef BK_QUAD_MINW_OTHER
+#define BK_QUAD_MINW_OTHER 1
+#endif
+template <class A, int R>
+constexpr int quad_min_waves() {
+  return sizeof(A) == 8 && R == 4 ? BK_QUAD_MINW_R4_64 : BK_QUAD_MINW_OTHER;
+}
+
+template <class T, int R, bool COND, bool SHIFT>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(quad_min_waves<typename acc_of<T>::type, R>())))
+void star_sweep_quads(const StarArgs<T, R, COND> a) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[8 * QuadTile<R, sizeof(A)>::PL];
+  __shared__ A tile[quad_tile_planes<A, R>() * QuadTile<R, sizeof(A)>::PL];
   const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
   if (u >= a.nunits) return;  // uniform over the workgroup
   A c[6 * R + 1];
 #pragma unroll
   for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
-  quad_unit<T, R, COND>(a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], (int)a.len[u],
-                        a.mask + (size_t)u * a.kcol, c, a.theta, a.cond_mode, a.img_off, a.img,
-                        tile);
+  quad_unit<T, R, COND, SHIFT, quad_face_prefetch<A, R>()>(
+      a.adj, a.in, a.out, a.in_step, a.out_step, a.anchor[u], (int)a.len[u],
+      a.mask + (size_t)u * a.kcol, c, a.theta, a.cond_mode, a.img_off, a.img, tile);
 }
 
 // Batched quad schedule: one persistent launch serves every request of a
 // group table whose bricks come as a quad plan; work item = one quad unit of
 // one request, taken by a whole workgroup.
 template <class T, int R, bool COND>
-__global__ __launch_bounds__(256) void star_sweep_quads_batched(const BatchTable *tab_,
-                                                                uint32_t tick) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(quad_min_waves<typename acc_of<T>::type, R>())))
+void star_sweep_quads_batched(const BatchTable *tab_, uint32_t tick) {
   using A = typename acc_of<T>::type;
-  __shared__ A tile[8 * QuadTile<R, sizeof(A)>::PL];
+  __shared__ A tile[quad_tile_planes<A, R>() * QuadTile<R, sizeof(A)>::PL];
   const kconst_ptr<BatchTable> tab = (kconst_ptr<BatchTable>)tab_;
   const uint32_t nsegs = tab->nsegs;
   const uint32_t total = tab->prefix[nsegs];
@@ -839,7 +983,10 @@ hipError_t launch_quads_t(const SweepLaunch &L, const PlanView &P, hipStream_t s
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
   if (P.nunits == 0) return hipSuccess;
-  hipLaunchKernelGGL((star_sweep_quads<T, R, COND>), dim3(P.nunits), dim3(256), 0, s, a);
+  if (L.lane_shift)
+    hipLaunchKernelGGL((star_sweep_quads<T, R, COND, true>), dim3(P.nunits), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((star_sweep_quads<T, R, COND, false>), dim3(P.nunits), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

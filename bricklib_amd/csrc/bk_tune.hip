@@ -98,10 +98,11 @@ int build_grid(int layout, int dtype, const long cells[3], int ghost, TuneGrid &
   std::vector<uint32_t> grid(nbr);
   g.adj.resize(nbr * 27);
   BK_TRY(bk_init_grid(g.padded, ghost ? 0 : 1, grid.data(), g.adj.data()));
+  // the sweep order of the library's grids: 32 x 16 (i, j) brick tiles
   uint32_t n = 0;
-  BK_TRY(bk_tiled_list(g.nb, ghost, 32, 32, nullptr, &n));
+  BK_TRY(bk_tiled_list(g.nb, ghost, 32, 16, nullptr, &n));
   g.list.resize(n);
-  BK_TRY(bk_tiled_list(g.nb, ghost, 32, 32, g.list.data(), &n));
+  BK_TRY(bk_tiled_list(g.nb, ghost, 32, 16, g.list.data(), &n));
   g.nlist = n;
   std::vector<uint32_t> pairs;
   if (ghost) {
@@ -128,13 +129,14 @@ int build_grid(int layout, int dtype, const long cells[3], int ghost, TuneGrid &
 // the schedules of one layout, in a fixed order
 void candidates_of(int layout, int stencil, unsigned flags, std::vector<bk_tune_cand> &out) {
   const bool batch_only = (flags & BK_TUNE_BATCH_SCHEDULES) != 0;
-  auto add = [&](int shape, uint32_t kcol, int wg) {
+  auto add = [&](int shape, uint32_t kcol, int wg, int shift = 0) {
     bk_tune_cand c;
     std::memset(&c, 0, sizeof(c));
     c.layout = layout;
     c.plan_shape = shape;
     c.kcol = kcol;
     c.wg_bricks = wg;
+    c.lane_shift = shift;
     c.ms = 0.0f;
     out.push_back(c);
   };
@@ -155,6 +157,9 @@ void candidates_of(int layout, int stencil, unsigned flags, std::vector<bk_tune_
   }
   for (uint32_t k : {4u, 8u, 16u}) add(BK_PLAN_COLUMN, k, 0);
   for (uint32_t k : {1u, 2u, 4u, 8u}) add(BK_PLAN_QUAD, k, 0);
+  // i-taps by DPP lane shifts: single-grid quad kernels only
+  if (!batch_only)
+    for (uint32_t k : {1u, 2u, 4u, 8u}) add(BK_PLAN_QUAD, k, 0, 1);
 }
 
 struct TuneKey {
@@ -267,6 +272,7 @@ int bk_tune_sweep(int stencil, int dtype, const double *h_coeff, double theta, i
     auto run = [&](size_t q, int n) -> int {
       d.plan = plans[q].p;
       d.wg_bricks = mine[q].wg_bricks;
+      d.lane_shift = mine[q].lane_shift;
       return bk_sweeps(&d, static_cast<const uint32_t *>(g.d_pairs.p), g.npairs, n, 1, s);
     };
     // untimed: one run each (graph capture), then warm_s seconds of all

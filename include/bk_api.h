@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define BK_API_VERSION 2
+#define BK_API_VERSION 3
 
 enum bk_status {
   BK_OK = 0,
@@ -325,6 +325,10 @@ typedef struct bk_sweep_desc {
   int wg_bricks;          /* bricks (waves) per workgroup of the per-brick
                              kernels: 0 (default, 4), 1, 2, 4 or 8 (8: the
                              8x8x8 stars only) */
+  int lane_shift;         /* 1: the star/cond kernels of a quad plan take the
+                             i-direction taps from DPP lane shifts of the
+                             wave's registers (same bits as 0, which reads
+                             them from the LDS tile); quad plans only */
 } bk_sweep_desc;
 
 /* One stencil sweep: for every brick b in d_list (or in the plan) and
@@ -395,6 +399,7 @@ typedef struct bk_tune_cand {
   int plan_shape;  /* 0, BK_PLAN_COLUMN or BK_PLAN_QUAD */
   uint32_t kcol;   /* plan length (0 without a plan) */
   int wg_bricks;   /* bricks per workgroup without a plan (0 with one) */
+  int lane_shift;  /* bk_sweep_desc.lane_shift of the schedule */
   float ms;
 } bk_tune_cand;
 
@@ -416,8 +421,9 @@ typedef struct bk_tune_cand {
  * builds the grid in every selected layout the extents fit, runs every
  * schedule of that layout -- per brick with 1, 2, 4 or 8 bricks per
  * workgroup (8: 8x8x8 stars only; 4: the cube on 8x8x8), k-columns of
- * 4/8/16 bricks and 2x2 column quads of 1/2/4/8 levels (8x8x8 stars),
- * quads of 2/4/8 levels (8x8x8 cube) -- untimed for warm_s seconds, then times `steps` ping-pong sweeps
+ * 4/8/16 bricks and 2x2 column quads of 1/2/4/8 levels (8x8x8 stars; each
+ * also with lane_shift = 1 unless BK_TUNE_BATCH_SCHEDULES), quads of 2/4/8
+ * levels (8x8x8 cube) -- untimed for warm_s seconds, then times `steps` ping-pong sweeps
  * per candidate (hipGraph replays) in `rounds` interleaved rounds, keeping
  * each candidate's best.  h_out receives up to max_out candidates fastest
  * first (h_out[0] is the choice); with h_out NULL *nout is the candidate

@@ -155,3 +155,56 @@ def test_fused_sweeps_argument_errors(bk):
     assert lib.bk_sweeps_fused(C.byref(d), None, None, None, 0, 2, 0, None) == BK_ERR_ARG
     d.stencil = 3  # the cube has no fused refresh
     assert lib.bk_sweeps_fused(C.byref(d), 64, 128, None, 0, 2, 0, None) == BK_ERR_UNSUPPORTED
+
+
+def test_lane_shift_argument_errors(bk):
+    """lane_shift takes 0 or 1 and is built into the quad star kernels
+    only: without a plan it is rejected before any HIP call."""
+    from bricklib_amd._lib import BK_ERR_ARG, BK_ERR_UNSUPPORTED, bk_sweep_desc
+    lib = bk.load()
+    d = bk_sweep_desc()
+    coeff = np.ones(7)
+    d.h_coeff = coeff.ctypes.data_as(C.POINTER(C.c_double))
+    d.nlist = 4
+    d.d_adj = d.d_list = d.d_in = 4096
+    d.d_out = 8192
+    d.in_step = d.out_step = 512
+    d.lane_shift = 2
+    assert lib.bk_sweep(C.byref(d), None) == BK_ERR_ARG
+    assert b"lane_shift" in lib.bk_last_error()
+    d.lane_shift = 1
+    assert lib.bk_sweep(C.byref(d), None) == BK_ERR_UNSUPPORTED
+    assert b"quad plans" in lib.bk_last_error()
+
+
+def test_struct_layouts_match_the_header(bk, tmp_path):
+    """The ctypes structures are the C structures of include/bk_api.h:
+    sizes and field offsets from a C program compiled against the header,
+    and the library's ABI version is the binding's."""
+    import subprocess
+    from bricklib_amd import _lib
+    structs = {"bk_sweep_desc": _lib.bk_sweep_desc, "bk_tune_cand": _lib.bk_tune_cand,
+               "bk_serve_config": _lib.bk_serve_config, "bk_batch_req": _lib.bk_batch_req}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bk_api.h"',
+             'int main(void) {', 'printf("version %d\\n", BK_API_VERSION);']
+    for s, cls in structs.items():
+        lines.append(f'printf("{s} size %zu\\n", sizeof({s}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{s} {f} %zu\\n", offsetof({s}, {f}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            *key, val = line.split()
+            got[tuple(key)] = int(val)
+    assert got[("version",)] == _lib.API_VERSION == bk.load().bk_version()
+    for s, cls in structs.items():
+        assert got[(s, "size")] == C.sizeof(cls), s
+        for f, _ in cls._fields_:
+            assert got[(s, f)] == getattr(cls, f).offset, (s, f)

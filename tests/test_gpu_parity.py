@@ -424,6 +424,67 @@ def test_fused_refresh_quads_match_oracle(gpu, orc, name, kcol):
                 np.testing.assert_array_equal(got[pr[:, 0]], got[pr[:, 1]])
 
 
+@pytest.mark.parametrize("stencil", [S.BK_STAR7, S.BK_STAR13, S.BK_STAR25, S.BK_COND7],
+                         ids=lambda s: NAMES[s])
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32", "bf16"])
+@pytest.mark.parametrize("kcol", [1, 3, 8])
+def test_lane_shift_quads_bit_exact(gpu, stencil, dtype, kcol):
+    """The quad kernels with the i-taps from DPP lane shifts (lane_shift=1)
+    give the oracle's bits, on grids whose extents leave partial quads and
+    levels, including single-brick axes (a member that is its own
+    i-neighbour); bricks outside the interior are never written."""
+    st = S.Stencil.make(stencil, seed=19)
+    for nb in ((3, 4, 5), (1, 3, 2), (2, 1, 4)):
+        print(f"[case] lane_shift {NAMES[stencil]} dtype={dtype} kcol={kcol} nb={nb}",
+              flush=True)
+        g, a, b, og, oa = setup(gpu, nb, dtype, 23)
+        g.ghost_refresh(a)
+        gpu.sweep(st, b, a, g, plan=g.plan("quad", kcol), lane_shift=1)
+        og.refresh(oa, dtype)
+        ob = og.storage(dtype)
+        og.sweep(stencil, st.coeff, oa, ob, dtype=dtype, theta=st.theta, cond_mode=st.cond_mode)
+        np.testing.assert_array_equal(host(b)[og.interior], ob[og.interior])
+        ghosts = np.setdiff1d(np.arange(og.nbricks), og.interior)
+        assert not host(b)[ghosts].any()
+
+
+@pytest.mark.parametrize("name", ["7pt", "25pt", "cond"])
+def test_lane_shift_fused_sweeps_match_oracle(gpu, name):
+    """Several sweeps with lane shifts and the fused refresh, eagerly and
+    replayed as a graph: the oracle's result, bit for bit."""
+    st = S.Stencil.make(name, seed=41)
+    nb = (3, 2, 5)
+    g = gpu.PeriodicGrid(nb)
+    og = O.Grid(nb)
+    for dt in (BK_F64, BK_F32):
+        for n, graph in ((1, False), (5, False), (4, True)):
+            print(f"[case] lane_shift fused {name} dt={dt} n={n} graph={graph}", flush=True)
+            a = gpu.Brick(g.info, g.field_storage(dt), 0)
+            b = gpu.Brick(g.info, g.field_storage(dt), 0)
+            g.fill_uniform(a, 6)
+            res = gpu.sweeps(st, a, b, g, n, use_graph=graph, plan=g.plan("quad", 2), fused=True,
+                             lane_shift=1)
+            got = res.storage.to_host()
+            oa, ob = og.storage(dt), og.storage(dt)
+            og.fill_uniform(oa, 6, dtype=dt)
+            want = og.sweeps(st.kind, st.coeff, oa, ob, n, dtype=dt, theta=st.theta,
+                             cond_mode=st.cond_mode)
+            np.testing.assert_array_equal(got[og.interior], want[og.interior])
+
+
+def test_lane_shift_rejected_outside_quad_stars(gpu):
+    from bricklib_amd import BrickError
+    g = gpu.PeriodicGrid((2, 2, 2))
+    a = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    b = gpu.Brick(g.info, g.field_storage(BK_F64), 0)
+    with pytest.raises(BrickError):
+        gpu.sweep(S.Stencil.make("125pt", seed=1), b, a, g, plan=g.plan("quad", 2), lane_shift=1)
+    with pytest.raises(BrickError):
+        gpu.sweep(S.Stencil.make("7pt", seed=1), b, a, g, plan=g.plan("column", 4), lane_shift=1)
+    with pytest.raises(BrickError):
+        gpu.sweep(S.Stencil.make("7pt", seed=1), b, a, g, plan=None, lane_shift=1)
+
+
 def test_fused_refresh_rejects_plans_and_cube(gpu):
     from bricklib_amd import BrickError
     g = gpu.PeriodicGrid((2, 2, 2))
@@ -450,7 +511,8 @@ def test_tuner_picks_a_schedule_with_the_same_bits(gpu, name):
     else:
         want = {"8x8x8 brick wg1", "8x8x8 brick wg2", "8x8x8 brick wg4", "8x8x8 brick wg8",
                 "8x8x8 column4", "8x8x8 column8", "8x8x8 column16", "8x8x8 quad1",
-                "8x8x8 quad2", "8x8x8 quad4", "8x8x8 quad8"}
+                "8x8x8 quad2", "8x8x8 quad4", "8x8x8 quad8", "8x8x8 quad1+dpp",
+                "8x8x8 quad2+dpp", "8x8x8 quad4+dpp", "8x8x8 quad8+dpp"}
     assert set(out["ms"]) == want
     assert out["best"] in out["ms"]
     g.fill_uniform(a, 2)
