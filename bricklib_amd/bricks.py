@@ -488,8 +488,8 @@ class PeriodicGrid:
             return None
         if int(stencil_kind) == _lib.BK_CUBE125 and self.interior.size > (1 << 20):
             # large grids: 8 levels (1024^3 FP64: 4.76 ms per sweep on one
-            # host against 5.0-6.2 with 4; on hosts whose clock drops under
-            # the long FP64 kernel every schedule lands at 5.8-6.1 ms)
+            # host against 5.0-6.2 with 4; on most hosts every schedule lands
+            # at 5.8-6.1 ms, and 8 levels were never the slower)
             choice = ("quad", 8)
         return self.plan(*choice)
 
