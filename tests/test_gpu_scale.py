@@ -64,13 +64,13 @@ def rel_rms_floor(got, ref):
     return float(np.max(np.abs(got - ref) / den)), rms
 
 
-def gpu_run(gpu, nb, st, dtype, seed, nsweeps):
+def gpu_run(gpu, nb, st, dtype, seed, nsweeps, **kw):
     g = gpu.PeriodicGrid(nb)
     a = gpu.Brick(g.info, g.field_storage(dtype), 0)
     b = gpu.Brick(g.info, g.field_storage(dtype), 0)
     g.fill_uniform(a, seed)
     b.storage.buf.zero()
-    res = gpu.sweeps(st, a, b, g, nsweeps)
+    res = gpu.sweeps(st, a, b, g, nsweeps, **kw)
     return g, res.storage.to_host()
 
 
@@ -143,6 +143,28 @@ def test_one_sweep_512_fp64_bit_exact(gpu, orc, case):
     e = O.field_compare(got, rref, og.interior, floor=rms)
     print(f"[err] {case_id(case)} 512^3 one sweep vs reference order: {e['max_rel']:.3e}")
     assert e["max_rel"] <= 1e-12
+
+
+@pytest.mark.parametrize("kind,dtype", [(S.BK_STAR7, BK_F64), (S.BK_STAR25, BK_F64),
+                                        (S.BK_COND7, BK_F64), (S.BK_STAR25, BK_F32)],
+                         ids=["7pt-f64", "25pt-f64", "cond-f64", "25pt-f32"])
+def test_one_sweep_512_lane_shift_bit_exact(gpu, orc, kind, dtype):
+    """The block kernels with the i-taps from DPP lane shifts at full
+    512^3, with the refresh fused into the sweep: the oracle's bits."""
+    print(f"[case] lane_shift one fused sweep 512^3 {NAMES[kind]} dtype={dtype}", flush=True)
+    st = make_stencil(kind, 1, seed=6)
+    nb = (64, 64, 64)
+    g, got = gpu_run(gpu, nb, st, dtype, 78, 1, fused=True, lane_shift=1)
+    og = O.Grid(nb)
+    oa = og.storage(dtype)
+    og.fill_uniform(oa, 78, dtype=dtype)
+    ob = og.storage(dtype)
+    ores = og.sweeps(kind, st.coeff, oa, ob, 1, dtype=dtype, theta=st.theta,
+                     cond_mode=st.cond_mode)
+    assert np.array_equal(got[og.interior], ores[og.interior])
+    # the fused refresh wrote the result's ghost bricks too
+    pr = g.pairs.reshape(-1, 2)
+    assert np.array_equal(got[pr[:, 0]], got[pr[:, 1]])
 
 
 # ---------------------------------------------------------------------------
