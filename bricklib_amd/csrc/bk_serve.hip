@@ -215,9 +215,19 @@ int schedule_of(bk_server_s *s, const Req &r, int *shape, uint32_t *kcol) {
   if (s->cfg.tune) {
     const int c = size_class(r.spec.nb);
     const auto key = std::make_tuple(c, st, r.spec.dtype);
-    auto it = s->tuned.find(key);
-    if (it == s->tuned.end()) {
-      // a cubic grid of about 2^c bricks
+    bk_tune_cand pick;
+    bool have = false;
+    {
+      // the map is read by bk_server_stats on other threads
+      std::lock_guard<std::mutex> lock(s->mu);
+      auto it = s->tuned.find(key);
+      if (it != s->tuned.end()) {
+        pick = it->second;
+        have = true;
+      }
+    }
+    if (!have) {
+      // a cubic grid of about 2^c bricks, timed without holding the lock
       const long e = std::max(1L, std::lround(std::cbrt(std::ldexp(1.0, c))));
       const long cells[3] = {8 * e, 8 * e, 8 * e};
       bk_tune_cand best[16];
@@ -227,10 +237,12 @@ int schedule_of(bk_server_s *s, const Req &r, int *shape, uint32_t *kcol) {
                            BK_TUNE_LAYOUT_8x8x8 | BK_TUNE_BATCH_SCHEDULES, best, 16, &n,
                            s->streams[0]));
       if (n == 0) return sfail(BK_ERR_HIP, "bk_server: tuning returned no candidate");
-      it = s->tuned.emplace(key, best[0]).first;
+      pick = best[0];
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->tuned.emplace(key, pick);
     }
-    *shape = it->second.plan_shape;
-    *kcol = it->second.kcol;
+    *shape = pick.plan_shape;
+    *kcol = pick.kcol;
     return BK_OK;
   }
   if (st == BK_CUBE125) {
@@ -270,9 +282,7 @@ int admit(bk_server_s *s, Req &r) {
   // filled on its lane's stream, which runs all its ticks
   BK_TRY(bk_fill_uniform_grid(r.spec.dtype, r.f0, 512, r.spec.nb, s->cfg.ghost, r.spec.seed,
                               -1.0, 1.0, s->streams[r.lane]));
-  r.parity = 0;
-  r.done = 0;
-  return BK_OK;
+  return BK_OK;  // done and parity start at 0 (a request is admitted once)
 }
 
 void release_fields(bk_server_s *s, Req &r) {
@@ -334,6 +344,8 @@ int run_chunk(bk_server_s *s, std::vector<Req *> &done) {
     if (recs.empty()) continue;
     BK_TRY(bk_batch_set(s->batches[l], recs.data(), (uint32_t)recs.size(), s->streams[l]));
     BK_TRY(bk_batch_run(s->batches[l], ticks, s->cfg.use_graph, s->streams[l]));
+    // read by bk_server_info on other threads
+    std::lock_guard<std::mutex> lock(s->mu);
     for (size_t q = 0; q < mine.size(); ++q) {
       mine[q]->done += recs[q].ticks;
       mine[q]->parity ^= recs[q].ticks & 1u;
@@ -382,9 +394,10 @@ int step(bk_server_s *s, bool *idle) {
     s->peak = std::max(s->peak, used);
   }
   for (Req *r : admit_now) {
-    BK_TRY(admit(s, *r));
+    BK_TRY(admit(s, *r));  // fields, plan and fill, outside the lock
+    std::lock_guard<std::mutex> lock(s->mu);
     r->admitted = s->now();
-    r->state = kActive;
+    r->state = kActive;  // publishes what admit wrote to bk_server_info's readers
     s->active.push_back(r);
   }
   Inflight &prev = s->slots[s->cur_slot ^ 1];
@@ -553,8 +566,12 @@ int bk_server_submit(bk_server s, const bk_serve_req *q, uint32_t *id) {
 
 int bk_server_prewarm(bk_server s, const bk_serve_req *qs, uint32_t n) {
   if (!s || (n && !qs)) return sfail(BK_ERR_ARG, "bk_server_prewarm: null pointer");
-  std::lock_guard<std::mutex> lock(s->mu);
-  if (s->running) return sfail(BK_ERR_ARG, "bk_server_prewarm: the server is running");
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->running) return sfail(BK_ERR_ARG, "bk_server_prewarm: the server is running");
+  }
+  // shapes and plans are the loop's own tables: no thread serves meanwhile
+  // (the caller's contract), so only the tuned map takes the lock
   for (uint32_t q = 0; q < n; ++q) {
     BK_TRY(check_spec(&qs[q]));
     Req r;
@@ -646,8 +663,10 @@ int bk_server_info(bk_server s, uint32_t id, bk_serve_info *out) {
   out->sweeps_done = r.done;
   out->field_bytes = r.field_bytes;
   out->d_result = r.state == kDone ? (r.parity ? r.f1 : r.f0) : nullptr;
-  out->plan_shape = r.plan ? bk::plan_view(r.plan).shape : 0;
-  out->kcol = r.plan ? bk::plan_view(r.plan).kcol : 0;
+  // the plan is written by admission before the state leaves kPending
+  const bool admitted = r.state >= kActive && r.plan;
+  out->plan_shape = admitted ? bk::plan_view(r.plan).shape : 0;
+  out->kcol = admitted ? bk::plan_view(r.plan).kcol : 0;
   return BK_OK;
 }
 

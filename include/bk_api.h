@@ -549,6 +549,11 @@ typedef struct bk_serve_stats {
   uint32_t tuned_classes;
 } bk_serve_stats;
 
+/* Threads: submit, wait, info, result, release, stats and reset_stats may
+ * be called from any thread at any time.  prewarm, run, start and stop are
+ * the server's driver: call them from one thread, and prewarm and run only
+ * while the server's own thread is not running.  result and release of the
+ * same request must not overlap (release returns the fields result reads). */
 int bk_server_create(const bk_serve_config *cfg, bk_server *out);
 /* Stops the thread (bk_server_stop), frees every field and table. */
 int bk_server_destroy(bk_server s);

@@ -50,9 +50,36 @@ def test_threaded_server_submit_while_running(gpu):
         r._server, r._sid, r.ghost = srv, sid.value, srv.ghost
         ids.append(sid.value)
     print("[case] threaded server: submitted", ids, flush=True)
-    for sid in ids:
-        rc = lib.bk_server_wait(h, sid, 60.0)
-        assert rc == 0, lib.bk_last_error()
+    # another thread polls every request's state and the statistics while
+    # the server's thread sweeps (ctypes calls release the GIL): the states
+    # and sweep counts it sees only move forward
+    import threading
+    seen, stop_poll, errors = {}, threading.Event(), []
+
+    def poll():
+        inf, st = _lib.bk_serve_info(), _lib.bk_serve_stats()
+        while not stop_poll.is_set():
+            for sid in ids:
+                if lib.bk_server_info(h, sid, C.byref(inf)) != 0:
+                    errors.append(sid)
+                    continue
+                prev = seen.get(sid, (0, 0))
+                if inf.state < prev[0] or inf.sweeps_done < prev[1]:
+                    errors.append((sid, prev, (inf.state, inf.sweeps_done)))
+                seen[sid] = (inf.state, inf.sweeps_done)
+            if lib.bk_server_stats(h, C.byref(st)) != 0:
+                errors.append("stats")
+
+    poller = threading.Thread(target=poll)
+    poller.start()
+    try:
+        for sid in ids:
+            rc = lib.bk_server_wait(h, sid, 60.0)
+            assert rc == 0, lib.bk_last_error()
+    finally:
+        stop_poll.set()
+        poller.join()
+    assert not errors, errors
     assert lib.bk_server_stop(h) == 0
     info = _lib.bk_serve_info()
     for r in reqs:
