@@ -668,8 +668,9 @@ __global__ __launch_bounds__(64) void cube_sweep_batched(const BatchTable *tab_,
   }
 }
 
+// the persistent grid: as many one-wave workgroups as fit on the device
 template <class T>
-hipError_t launch_cube_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+int cube_batched_blocks() {
   static int nblk = 0;
   if (nblk == 0) {
     int dev = 0, cus = 256, per = 1;
@@ -678,7 +679,13 @@ hipError_t launch_cube_batched_t(const BatchTable *tab, uint32_t tick, hipStream
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, cube_sweep_batched<T>, 64, 0);
     nblk = cus * (per > 0 ? per : 1);
   }
-  hipLaunchKernelGGL((cube_sweep_batched<T>), dim3(nblk), dim3(64), 0, s, tab, tick);
+  return nblk;
+}
+
+template <class T>
+hipError_t launch_cube_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  hipLaunchKernelGGL((cube_sweep_batched<T>), dim3(cube_batched_blocks<T>()), dim3(64), 0, s,
+                     tab, tick);
   return hipGetLastError();
 }
 
@@ -731,6 +738,15 @@ hipError_t launch_cube(const SweepLaunch &L, hipStream_t s) {
     case BK_F32: return launch_t<float>(L, s);
     case BK_BF16: return launch_t<bf16>(L, s);
     default: return hipErrorInvalidValue;
+  }
+}
+
+uint32_t cube_batched_capacity(int dtype) {
+  switch (dtype) {
+    case BK_F64: return (uint32_t)cube_batched_blocks<double>();
+    case BK_F32: return (uint32_t)cube_batched_blocks<float>();
+    case BK_BF16: return (uint32_t)cube_batched_blocks<bf16>();
+    default: return 0;
   }
 }
 

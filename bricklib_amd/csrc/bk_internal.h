@@ -109,6 +109,8 @@ hipError_t launch_star_columns_batched(int stencil, int dtype, const BatchTable 
 hipError_t launch_star_quads_batched(int stencil, int dtype, const BatchTable *tab,
                                      uint32_t tick, hipStream_t s);
 hipError_t launch_cube_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
+// quad units the persistent batched cube kernel holds at once (one per wave)
+uint32_t cube_batched_capacity(int dtype);
 hipError_t launch_ghost_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
 
 hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,

@@ -506,7 +506,10 @@ typedef struct bk_serve_config {
   int ghost;            /* 0: wrapped adjacency (no refresh), 1: ghost layer */
   int pipeline;         /* queue the next chunk before waiting for the last */
   int tune;             /* schedules from bk_tune_sweep per (size class,
-                           stencil, dtype); 0: fixed defaults */
+                           stencil, dtype); 0: fixed defaults -- for cubes
+                           quads of 2 levels, or of 1 level in a chunk where
+                           the lane's cube units of that dtype number fewer
+                           than four times what the batched kernel holds */
   size_t slab_bytes;    /* arena slab size (0: 1 GiB) */
   int star_plan;        /* fixed default for stars and cond: 0 one brick
                            per wave, BK_PLAN_QUAD 2x2 brick blocks (levels
@@ -533,7 +536,8 @@ typedef struct bk_serve_info {
   uint32_t sweeps_done;
   size_t field_bytes;
   const void *d_result; /* state 2: the field holding the result */
-  int plan_shape;       /* the schedule it ran: 0 (per brick) or a plan shape */
+  int plan_shape;       /* the schedule of its last chunk: 0 (per brick) or a
+                           plan shape, and the plan's levels per unit */
   uint32_t kcol;
 } bk_serve_info;
 

@@ -154,5 +154,23 @@ def test_star_plan_choices_give_the_same_bits(gpu, star_plan, ghost):
     for r in reqs:
         want = {None: 0, "column": _lib.BK_PLAN_COLUMN, "quad": _lib.BK_PLAN_QUAD}[star_plan]
         assert r.schedule[0] == (_lib.BK_PLAN_QUAD if r.stencil.kind == 3 else want)
+        if r.stencil.kind == 3:
+            # a light load: the cubes ran one-level quads
+            assert r.schedule[1] == 1
         print(f"[case] star_plan={star_plan} ghost={ghost} rid={r.rid}", flush=True)
         np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
+
+
+def test_heavy_cube_load_runs_two_level_quads(gpu):
+    """A cube load far above what the batched kernel holds at once (a 512^3
+    FP64 grid: 32768 two-level units, at least four times the resident
+    waves of any occupancy) runs two-level quads, with the single-grid
+    bits."""
+    r = Request(rid=0, nb=(64, 64, 64), stencil=Stencil.make("125pt", seed=47), dtype=BK_F64,
+                steps=2, seed=48)
+    srv = serve.Server(chunk=2)
+    stats = srv.serve([r], realtime=False, keep_results=True)
+    assert len(stats.completed) == 1
+    print(f"[case] heavy cube schedule={r.schedule}", flush=True)
+    assert r.schedule == (_lib.BK_PLAN_QUAD, 2)
+    np.testing.assert_array_equal(r.interior_host(), alone(gpu, r))
