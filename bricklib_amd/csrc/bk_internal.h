@@ -111,6 +111,8 @@ hipError_t launch_star_quads_batched(int stencil, int dtype, const BatchTable *t
 hipError_t launch_cube_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
 // quad units the persistent batched cube kernel holds at once (one per wave)
 uint32_t cube_batched_capacity(int dtype);
+// quad units the persistent batched star kernel holds at once (one per workgroup)
+uint32_t star_quads_batched_capacity(int stencil, int dtype);
 hipError_t launch_ghost_batched(int dtype, const BatchTable *tab, uint32_t tick, hipStream_t s);
 
 hipError_t launch_ghost_refresh(void *dat, size_t step_bytes,

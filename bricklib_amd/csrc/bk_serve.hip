@@ -135,7 +135,7 @@ struct Req {
   int state = kPending;
   Shape *shape = nullptr;
   bk_plan plan = nullptr;        // the schedule chosen at admission
-  bk_plan plan_light = nullptr;  // cubes with the default schedule: one-level quads
+  bk_plan plan_light = nullptr;  // default quad schedules of several levels: half as many
   bk_plan plan_used = nullptr;   // the plan of its last chunk
   void *f0 = nullptr, *f1 = nullptr;
   size_t field_bytes = 0;
@@ -248,8 +248,7 @@ int schedule_of(bk_server_s *s, const Req &r, int *shape, uint32_t *kcol) {
     return BK_OK;
   }
   if (st == BK_CUBE125) {
-    // quads of two levels; run_chunk switches to one level while the lane's
-    // cube work is light (cube_levels)
+    // quads of two levels (one while the load is light: quad_levels)
     *shape = BK_PLAN_QUAD;
     *kcol = 2;
   } else if (s->cfg.star_plan == BK_PLAN_QUAD) {
@@ -276,8 +275,8 @@ int admit(bk_server_s *s, Req &r) {
   r.plan = nullptr;
   if (shape) BK_TRY(r.shape->plan(shape, kcol, &r.plan));
   r.plan_light = nullptr;
-  if (r.spec.stencil == BK_CUBE125 && !s->cfg.tune)
-    BK_TRY(r.shape->plan(BK_PLAN_QUAD, 1, &r.plan_light));
+  if (shape == BK_PLAN_QUAD && kcol > 1 && !s->cfg.tune)
+    BK_TRY(r.shape->plan(BK_PLAN_QUAD, kcol / 2, &r.plan_light));
   r.lane = s->lane_of(r.spec.stencil, r.spec.dtype);
   BK_TRY(bk_arena_alloc(s->arena, r.field_bytes, &r.f0));
   int rc = bk_arena_alloc(s->arena, r.field_bytes, &r.f1);
@@ -317,19 +316,26 @@ void retire(bk_server_s *s, std::vector<Req *> &done) {
   s->cv.notify_all();
 }
 
-// Cubes with the default schedule run two-level quads while a lane's cube
-// work of one dtype fills the persistent kernel's resident waves four times
-// over, and one-level quads below that: a tick of a light load lasts one
-// unit, so halving the unit halves the tick (Poisson p99 29.6 -> 21.7 ms),
-// while a heavy load spends less on halo planes with longer units
-// (saturating 76.3 % against 74.9 % of the single-grid rates).
-void cube_levels(bk_server_s *s, int lane, std::map<int, bool> &light) {
-  std::map<int, uint64_t> units;
+// Requests with a default quad schedule of several levels run it while
+// their lane's units of one (stencil, dtype) group fill that group's
+// persistent kernel four times over, and units of half the levels below
+// that: a tick of a light load lasts about one unit, so halving the unit
+// halves the tick, while a heavy load spends less on halo planes with
+// longer units.  Cubes of 2 and 1 levels: Poisson p99 29.6 -> 21.7 ms with
+// one level throughout, saturating 76.3 % of the single-grid rates with two
+// throughout against 74.9 % with one.
+using GroupKey = std::pair<int, int>;  // (stencil, dtype)
+void quad_levels(bk_server_s *s, int lane, std::map<GroupKey, bool> &light) {
+  std::map<GroupKey, uint64_t> units;
   for (Req *r : s->active)
     if (r->lane == lane && r->plan_light)
-      units[r->spec.dtype] += bk::plan_view(r->plan).nunits;
-  for (auto &u : units)
-    light[u.first] = u.second < 4ull * bk::cube_batched_capacity(u.first);
+      units[{r->spec.stencil, r->spec.dtype}] += bk::plan_view(r->plan).nunits;
+  for (auto &u : units) {
+    const int st = u.first.first, dt = u.first.second;
+    const uint32_t cap = st == BK_CUBE125 ? bk::cube_batched_capacity(dt)
+                                          : bk::star_quads_batched_capacity(st, dt);
+    light[u.first] = u.second < 4ull * cap;
+  }
 }
 
 // one chunk of ticks over the active requests, lane by lane
@@ -338,8 +344,8 @@ int run_chunk(bk_server_s *s, std::vector<Req *> &done) {
   for (Req *r : s->active) ticks = std::max(ticks, r->remaining());
   ticks = std::min(ticks, s->cfg.chunk);
   for (int l = 0; l < s->cfg.lanes; ++l) {
-    std::map<int, bool> light;
-    cube_levels(s, l, light);
+    std::map<GroupKey, bool> light;
+    quad_levels(s, l, light);
     std::vector<bk_batch_req> recs;
     std::vector<Req *> mine;
     for (Req *r : s->active) {
@@ -351,7 +357,7 @@ int run_chunk(bk_server_s *s, std::vector<Req *> &done) {
       q.d_adj = static_cast<const uint32_t *>(r->shape->d_adj);
       q.d_list = static_cast<const uint32_t *>(r->shape->d_list);
       q.nlist = r->shape->nlist;
-      q.plan = r->plan_light && light[r->spec.dtype] ? r->plan_light : r->plan;
+      q.plan = r->plan_light && light[{r->spec.stencil, r->spec.dtype}] ? r->plan_light : r->plan;
       q.d_pairs = static_cast<const uint32_t *>(r->shape->d_pairs);
       q.npairs = r->shape->npairs;
       q.d_f0 = r->f0;
@@ -608,8 +614,8 @@ int bk_server_prewarm(bk_server s, const bk_serve_req *qs, uint32_t n) {
     BK_TRY(schedule_of(s, r, &shape, &kcol));
     bk_plan p = nullptr;
     if (shape) BK_TRY(r.shape->plan(shape, kcol, &p));
-    if (r.spec.stencil == BK_CUBE125 && !s->cfg.tune)
-      BK_TRY(r.shape->plan(BK_PLAN_QUAD, 1, &p));
+    if (shape == BK_PLAN_QUAD && kcol > 1 && !s->cfg.tune)
+      BK_TRY(r.shape->plan(BK_PLAN_QUAD, kcol / 2, &p));
   }
   return bk_device_sync();
 }

@@ -1140,7 +1140,7 @@ hipError_t launch_cols_batched_t(const BatchTable *tab, uint32_t tick, hipStream
 }
 
 template <class T, int R, bool COND>
-hipError_t launch_quads_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+int quads_batched_blocks() {
   static int nblk = 0;
   if (nblk == 0) {
     int dev = 0, cus = 256, per = 1;
@@ -1150,9 +1150,25 @@ hipError_t launch_quads_batched_t(const BatchTable *tab, uint32_t tick, hipStrea
         &per, star_sweep_quads_batched<T, R, COND>, 256, 0);
     nblk = cus * (per > 0 ? per : 1);
   }
-  hipLaunchKernelGGL((star_sweep_quads_batched<T, R, COND>), dim3(nblk), dim3(256), 0, s, tab,
-                     tick);
+  return nblk;
+}
+
+template <class T, int R, bool COND>
+hipError_t launch_quads_batched_t(const BatchTable *tab, uint32_t tick, hipStream_t s) {
+  hipLaunchKernelGGL((star_sweep_quads_batched<T, R, COND>),
+                     dim3(quads_batched_blocks<T, R, COND>()), dim3(256), 0, s, tab, tick);
   return hipGetLastError();
+}
+
+template <class T>
+uint32_t quads_batched_capacity_dt(int stencil) {
+  switch (stencil) {
+    case BK_STAR7: return (uint32_t)quads_batched_blocks<T, 1, false>();
+    case BK_STAR13: return (uint32_t)quads_batched_blocks<T, 2, false>();
+    case BK_STAR25: return (uint32_t)quads_batched_blocks<T, 4, false>();
+    case BK_COND7: return (uint32_t)quads_batched_blocks<T, 1, true>();
+    default: return 0;
+  }
 }
 
 template <class T>
@@ -1258,6 +1274,15 @@ hipError_t launch_star_columns_batched(int stencil, int dtype, const BatchTable 
     case BK_F32: return launch_cols_batched_dt<float>(stencil, tab, tick, s);
     case BK_BF16: return launch_cols_batched_dt<bf16>(stencil, tab, tick, s);
     default: return hipErrorInvalidValue;
+  }
+}
+
+uint32_t star_quads_batched_capacity(int stencil, int dtype) {
+  switch (dtype) {
+    case BK_F64: return quads_batched_capacity_dt<double>(stencil);
+    case BK_F32: return quads_batched_capacity_dt<float>(stencil);
+    case BK_BF16: return quads_batched_capacity_dt<bf16>(stencil);
+    default: return 0;
   }
 }
 

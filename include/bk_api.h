@@ -506,10 +506,12 @@ typedef struct bk_serve_config {
   int ghost;            /* 0: wrapped adjacency (no refresh), 1: ghost layer */
   int pipeline;         /* queue the next chunk before waiting for the last */
   int tune;             /* schedules from bk_tune_sweep per (size class,
-                           stencil, dtype); 0: fixed defaults -- for cubes
-                           quads of 2 levels, or of 1 level in a chunk where
-                           the lane's cube units of that dtype number fewer
-                           than four times what the batched kernel holds */
+                           stencil, dtype); 0: fixed defaults -- cubes in
+                           quads of 2 levels, stars per star_plan; a quad
+                           schedule of L > 1 levels runs L/2 in a chunk where
+                           the lane's units of that stencil and dtype number
+                           fewer than four times what the batched kernel
+                           holds at once */
   size_t slab_bytes;    /* arena slab size (0: 1 GiB) */
   int star_plan;        /* fixed default for stars and cond: 0 one brick
                            per wave, BK_PLAN_QUAD 2x2 brick blocks (levels
