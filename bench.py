@@ -305,8 +305,9 @@ def serve_bench(bk, S, args):
     # real-time Poisson stream
     reqs = SV.workload(args.serve_requests, rate=args.serve_rate, seed=2024)
     # chunks of 16 ticks: at this load a chunk's GPU time is short and the
-    # host's round trip per chunk adds to every request's latency (p50 10.5
-    # ms with chunks of 8, 8.5-9.0 with 16 or 32, 11 with 64)
+    # host's round trip per chunk adds to every request's latency (p50 / p99
+    # 6.2 / 24.4 ms with chunks of 8, 4.8 / 21.7 with 16, 4.4 / 22.5 with 32;
+    # the saturating run alike for all)
     srv = SV.Server(hbm_budget=200e9, chunk=16, ghost=serve_ghost, tune=args.serve_tune)
     t_pw = time.perf_counter()
     srv.prewarm(reqs)
