@@ -273,7 +273,8 @@ void check_layout() {
 
 template <typename BrickT>
 bk_sweep_desc make_desc(const BrickStencil &st, BrickT &bOut, BrickT &bIn,
-                        const unsigned *d_list, unsigned nlist, bk_plan plan) {
+                        const unsigned *d_list, unsigned nlist, bk_plan plan,
+                        int lane_shift = 0) {
   bkCheckVersion();
   check_layout<BrickT>();
   if (!bIn.bStorage.on_device || !bOut.bStorage.on_device)
@@ -294,17 +295,21 @@ bk_sweep_desc make_desc(const BrickStencil &st, BrickT &bOut, BrickT &bIn,
   d.theta = st.theta;
   d.cond_mode = st.cond_mode;
   d.plan = plan;
+  d.lane_shift = lane_shift;
   return d;
 }
 }  // namespace bk_detail
 
 /// One sweep bOut = stencil(bIn) over the nlist bricks of d_list (a device
 /// array) or over a plan's bricks.  bIn.bInfo must come from movBrickInfo.
+/// lane_shift = 1 with a BK_PLAN_QUAD plan of a star or cond stencil: the
+/// i-direction taps from DPP lane shifts (the same bits).
 template <typename BrickT>
 void brick_sweep(const BrickStencil &st, BrickT &bOut, BrickT &bIn, const unsigned *d_list,
-                 unsigned nlist, bk_stream s = nullptr, bk_plan plan = nullptr) {
+                 unsigned nlist, bk_stream s = nullptr, bk_plan plan = nullptr,
+                 int lane_shift = 0) {
   brick_flush();
-  bk_sweep_desc d = bk_detail::make_desc(st, bOut, bIn, d_list, nlist, plan);
+  bk_sweep_desc d = bk_detail::make_desc(st, bOut, bIn, d_list, nlist, plan, lane_shift);
   bkCheck(bk_sweep(&d, s), "brick_sweep");
 }
 
@@ -314,9 +319,10 @@ void brick_sweep(const BrickStencil &st, BrickT &bOut, BrickT &bIn, const unsign
 template <typename BrickT>
 void brick_sweeps(const BrickStencil &st, BrickT &a, BrickT &b, const unsigned *d_list,
                   unsigned nlist, const unsigned *d_pairs, unsigned npairs, int nsweeps,
-                  bool use_graph = false, bk_stream s = nullptr, bk_plan plan = nullptr) {
+                  bool use_graph = false, bk_stream s = nullptr, bk_plan plan = nullptr,
+                  int lane_shift = 0) {
   brick_flush();
-  bk_sweep_desc d = bk_detail::make_desc(st, b, a, d_list, nlist, plan);
+  bk_sweep_desc d = bk_detail::make_desc(st, b, a, d_list, nlist, plan, lane_shift);
   bkCheck(bk_sweeps(&d, d_pairs, npairs, nsweeps, use_graph ? 1 : 0, s), "brick_sweeps");
 }
 
