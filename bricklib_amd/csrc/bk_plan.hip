@@ -296,7 +296,7 @@ int bk_plan_create(const uint32_t *h_adj, uint32_t nbricks, const uint32_t *h_li
   p->ncovered = nc;
   p->nbricks = nbricks;
   p->id = g_next_plan_id.fetch_add(1);
-  hipGetDevice(&p->device);
+  (void)hipGetDevice(&p->device);  // the current device owns the arrays below
   auto up = [](uint32_t **dst, const std::vector<uint32_t> &v) -> hipError_t {
     if (v.empty()) return hipSuccess;
     hipError_t e = hipMalloc(dst, v.size() * sizeof(uint32_t));
@@ -321,11 +321,11 @@ int bk_plan_destroy(bk_plan p) {
   if (!p) return BK_OK;
   // cached graphs captured the plan's device arrays: drop them first
   bk::graphs_forget_plan(p->id);
-  hipFree(p->d_anchor);
-  hipFree(p->d_len);
-  hipFree(p->d_mask);
-  hipFree(p->d_flat);
-  hipFree(p->d_start);
+  (void)hipFree(p->d_anchor);
+  (void)hipFree(p->d_len);
+  (void)hipFree(p->d_mask);
+  (void)hipFree(p->d_flat);
+  (void)hipFree(p->d_start);
   delete p;
   return BK_OK;
 }

@@ -745,13 +745,13 @@ int sweeps_impl(const bk_sweep_desc *d, const uint32_t *d_pairs, uint32_t npairs
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e != hipSuccess) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       return hip_fail(e, "capture of sweep sequence");
     }
     if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
     hipGraphExec_t ge = nullptr;
     e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);  // the executable graph keeps what it needs
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
     BK_HIP(gc.exec.insert(key, ge, pid, apid));
   }
