@@ -16,10 +16,11 @@ The same JSON line also carries:
                   (16 B/point: one FP64 read + one write) / its mean launch
                   time, against the 8 TB/s HBM3E peak; `measured_copy_gbs`
                   is a plain device copy kernel on the same box;
-  cpu_baseline -- the oracle's restatement of the reference's CPU brick
-                  loops (stencils/3axis.cpp:51-66 and the reference-order
-                  13/25/125-point and cond loops) with OpenMP on the host
-                  cores, per stencil, with the CPU model and core counts;
+  cpu_baseline -- the reference's CPU brick loops with OpenMP on the host
+                  cores, per stencil, with the CPU model and core counts:
+                  its own 7-point and cond loops (stencils/3axis.cpp:51-66,
+                  :123-140) compiled from its headers, the oracle's
+                  restatement of the 13/25/125-point tap sets;
   by_stencil   -- 13/25-point, 125-point cube and cond at the same size,
                   the FP32 / BF16 forms of the 7-point star, FP32 25-point
                   and cube, and the 7/25-point block kernels with the
@@ -215,13 +216,18 @@ def host_cpu():
 
 
 def cpu_baseline(nb, seconds):
-    """The oracle's restatement of the reference's CPU brick loops on the
-    host, OpenMP over the interior bricks (stencils/3axis.cpp:51-66 for the
-    7-point star through the Brick accessor; the reference-order loops of
-    the 13/25/125-point and cond stencils), each timed like time_func
-    (stencils/stencils.h:40-53: one warm-up, then whole sweeps with the
-    periodic refresh until its share of `seconds` has passed), over the full
-    grid.  `value` is the 7-point star's rate."""
+    """The reference's CPU brick loops on the host, OpenMP over the interior
+    bricks, each timed like time_func (stencils/stencils.h:40-53: one
+    warm-up, then whole sweeps with the periodic refresh until its share of
+    `seconds` has passed), over the full grid.  The 7-point star and cond
+    run the reference's own loops (stencils/3axis.cpp:51-66, :123-140)
+    through its own Brick accessor, compiled from its headers (oracle/_ref;
+    kind "reference"); the 13/25/125-point stencils exist in the reference
+    only as generated code, so they run the oracle's restatement of their
+    tap sets in the reference's order (kind "port"), as do all five when
+    oracle/_ref is absent.  `value` is the 7-point star's rate."""
+    import ctypes as C
+
     import oracle as O
     from bricklib_amd import stencils as S
     g = O.Grid(nb)
@@ -229,16 +235,23 @@ def cpu_baseline(nb, seconds):
     b = g.storage()
     g.fill_uniform(a, 1234)
     g.refresh(a)
+    R = O.ref() if O.ref_available() else None
     acc = O.lib().or_brick_sweep_7pt_accessor
     pts = g.interior.size * 512
     by = {}
     names = ("7pt", "13pt", "25pt", "125pt", "cond")
     for name in names:
-        st = S.Stencil.make(name, seed=1)
+        # cond in the reference's form (p = sum c max(x, 0), |p|) for both kinds
+        st = S.Stencil.make(name, seed=1, cond_mode=0)
         c = np.ascontiguousarray(st.coeff)
+        ref_loop = R is not None and name in ("7pt", "cond")
 
         def sweep(src, dst):
-            if name == "7pt":
+            if ref_loop:
+                fn = R.ref_sweep_7pt if name == "7pt" else R.ref_sweep_cond
+                fn(O.P(g.adj), g.nbricks, O.P(g.interior), g.interior.size, O.P(src),
+                   C.c_size_t(512), O.P(dst), C.c_size_t(512), O.P(c))
+            elif name == "7pt":
                 acc(O.P(g.adj, O.u32p), O.P(g.interior, O.u32p), g.interior.size,
                     O.P(src, O.dblp), 512, O.P(dst, O.dblp), 512, O.P(c, O.dblp))
             else:
@@ -256,13 +269,17 @@ def cpu_baseline(nb, seconds):
             if el >= seconds / len(names) or n >= 1000:
                 break
         by[name] = {"gstencil_s": round(pts * n / el / 1e9, 4), "sweeps": n,
-                    "seconds": round(el, 2)}
+                    "seconds": round(el, 2), "kind": "reference" if ref_loop else "port"}
     info = host_cpu()
     threads = int(O.lib().or_num_threads())
+    kind = by["7pt"]["kind"]
     return {"value": by["7pt"]["gstencil_s"], "unit": "GStencil/s", "cores": threads,
-            "kind": "port",
+            "kind": kind,
             "sample": f"FP64 {8 * nb[0]}^3 grid, whole sweeps with the periodic refresh, "
-                      f"{seconds / len(names):.1f} s per stencil (value: the 7-point star)",
+                      f"{seconds / len(names):.1f} s per stencil (value: the 7-point star, "
+                      + ("the reference's stencils/3axis.cpp brick loop compiled from its "
+                         "headers" if kind == "reference" else "the oracle's restatement")
+                      + ")",
             "threads": threads, **info, "by_stencil": by}
 
 
