@@ -477,11 +477,14 @@ def main():
         print(f"[bench] measured VALU peak: {vpeak}", file=sys.stderr, flush=True)
         only = set(x for x in args.only.split(",") if x)
         # the *_lane_shift cases: the same default schedule (2x2 brick
-        # blocks) with the i-direction taps from DPP lane shifts
-        cases = [("13pt_f64", "13pt", bk.BK_F64, {}), ("25pt_f64", "25pt", bk.BK_F64, {}),
-                 ("125pt_f64", "125pt", bk.BK_F64, {}), ("cond_f64", "cond", bk.BK_F64, {}),
+        # blocks) with the i-direction taps from DPP lane shifts, each right
+        # after its LDS-tile twin (the clocks after the VALU-heavy cube and
+        # probe differ: compared across them, the two forms swap places)
+        cases = [("7pt_f64_tile_taps", "7pt", bk.BK_F64, {"lane_shift": 0}),
                  ("7pt_f64_lane_shift", "7pt", bk.BK_F64, {"lane_shift": 1}),
+                 ("13pt_f64", "13pt", bk.BK_F64, {}), ("25pt_f64", "25pt", bk.BK_F64, {}),
                  ("25pt_f64_lane_shift", "25pt", bk.BK_F64, {"lane_shift": 1}),
+                 ("cond_f64", "cond", bk.BK_F64, {}), ("125pt_f64", "125pt", bk.BK_F64, {}),
                  ("7pt_f32", "7pt", bk.BK_F32, {}), ("25pt_f32", "25pt", bk.BK_F32, {}),
                  ("25pt_f32_lane_shift", "25pt", bk.BK_F32, {"lane_shift": 1}),
                  ("7pt_bf16", "7pt", bk.BK_BF16, {}), ("125pt_f32", "125pt", bk.BK_F32, {})]
