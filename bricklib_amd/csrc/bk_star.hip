@@ -20,6 +20,8 @@
 //     over (i+r, i-r, j+r, j-r, k+r, k-r) for r = 1..R -- the tap order of
 //     the reference's stencil files, fixed so results are reproducible.
 // Neighbour bricks are found through the adjacency table in HBM.
+#include <type_traits>
+
 #include "bk_device.h"
 #include "bk_internal.h"
 
@@ -395,6 +397,223 @@ __global__ __launch_bounds__(64 * WPB) void star_sweep_lds(const StarArgs<T, R, 
   }
 }
 
+
+// BF16 storage in the row form: one wave per brick, lane L = k*8 + j holds
+// the brick's i-row (k, j) -- eight 2-byte cells -- as one 16-byte load and
+// writes it back as one 16-byte store.  (The column form above moves 2 bytes
+// per lane per plane for BF16: eight times the memory instructions per byte
+// of this form.)  The i taps are the lane's own cells plus R cells of the
+// same row of the i-1 / i+1 neighbours (one 4- or 8-byte load each); the j
+// and k taps are whole rows, of other lanes or of the j/k neighbours' R-deep
+// faces, exchanged through a per-wave LDS tile of rows at (kk + R, jj + R),
+// kk, jj = -R .. 7 + R (corners unused).  Each point's taps are applied in
+// the column form's order (c0 x, then i+r, i-r, j+r, j-r, k+r, k-r for
+// r = 1..R, FP32 FMAs, round to nearest even on store), so the bits are the
+// same.  Needs 16-byte aligned fields and brick steps (the host checks).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+// cell i of a row of eight BF16 cells
+__device__ __forceinline__ float bf16_cell(const u32x4 &row, int i) {
+  return (i & 1) ? bf16_hi(row[i >> 1]) : bf16_lo(row[i >> 1]);
+}
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  bf16 a, b;
+  narrow(a, lo);
+  narrow(b, hi);
+  return (uint32_t)a.u | ((uint32_t)b.u << 16);
+}
+
+template <int R>
+struct RowTile {
+  static constexpr int P = 8 + 2 * R;
+  __device__ static constexpr int at(int kk, int jj) { return (kk + R) * P + jj + R; }
+};
+
+// Bricks per wave of the row form: BPW consecutive list entries, all their
+// loads issued before the first is computed (more bytes in flight per wave).
+// One is the default: at 512^3 two ran the 7-point in 129-132 us against
+// 131 (the 25-point 203-217 against 194) and four in 136-141 (25-point 290).
+#ifndef BK_ROWS_BPW
+#define BK_ROWS_BPW 1
+#endif
+
+template <int R, bool COND>
+struct RowBrick {
+  static constexpr int NF = (32 * R + 63) / 64;  // face rows per lane
+  static constexpr int MAXIMG = 7;  // image ids read ahead (a periodic grid's corner brick has 7)
+  using TL = RowTile<R>;
+  uint32_t b = 0, nimg = 0, i0 = 0;
+  uint32_t img[MAXIMG];
+  u32x4 own;
+  u32x4 fr[NF];
+  int fdst[NF];
+  // the i taps past the brick's edges, as loaded: west cells 6, 7 and east
+  // cells 0, 1 (R <= 2) or west 4 .. 7 and east 0 .. 3 (R = 4)
+  using Edge = typename std::conditional<R <= 2, uint32_t, u32x2>::type;
+  Edge we, ee;
+
+  // every global load of brick b (list entry): its rows, the 32R face rows
+  // (face f / 8R: k-1, k+1, j-1, j+1; row q = f % 8R of it), the i edges and
+  // the fused refresh's image ids
+  __device__ __forceinline__ void load(const StarArgs<bf16, R, COND> &a, uint32_t bb, int lane) {
+    b = bb;
+    const int k = lane >> 3, j = lane & 7;
+    (void)k;
+    (void)j;
+    const uint32_t *ab = a.adj + (size_t)b * 27;
+    const uint16_t *in = reinterpret_cast<const uint16_t *>(a.in);
+    const size_t st = a.in_step;
+    own = *reinterpret_cast<const u32x4 *>(in + (size_t)b * st + lane * 8);
+    const uint16_t *pim = in + (size_t)uniform(ab[12]) * st + lane * 8;
+    const uint16_t *pip = in + (size_t)uniform(ab[14]) * st + lane * 8;
+    const uint16_t *pkm = in + (size_t)uniform(ab[4]) * st;
+    const uint16_t *pkp = in + (size_t)uniform(ab[22]) * st;
+    const uint16_t *pjm = in + (size_t)uniform(ab[10]) * st;
+    const uint16_t *pjp = in + (size_t)uniform(ab[16]) * st;
+#pragma unroll
+    for (int n = 0; n < NF; ++n) {
+      const int f = n * 64 + lane;
+      fdst[n] = -1;
+      if (f < 32 * R) {
+        const int face = f / (8 * R), q = f % (8 * R);
+        int sk, sj, kk, jj;
+        const uint16_t *src;
+        if (face < 2) {  // k faces: rows (kk, jj), kk = -R .. -1 or 8 .. 7 + R
+          sj = jj = q & 7;
+          sk = face == 0 ? 8 - R + (q >> 3) : q >> 3;
+          kk = face == 0 ? -R + (q >> 3) : 8 + (q >> 3);
+          src = face == 0 ? pkm : pkp;
+        } else {  // j faces: rows (kk, jj), jj = -R .. -1 or 8 .. 7 + R
+          sk = kk = q / R;
+          sj = face == 2 ? 8 - R + q % R : q % R;
+          jj = face == 2 ? -R + q % R : 8 + q % R;
+          src = face == 2 ? pjm : pjp;
+        }
+        fr[n] = *reinterpret_cast<const u32x4 *>(src + (sk * 8 + sj) * 8);
+        fdst[n] = TL::at(kk, jj);
+      }
+    }
+    we = *reinterpret_cast<const Edge *>(pim + (R <= 2 ? 6 : 4));
+    ee = *reinterpret_cast<const Edge *>(pip);
+    if (a.img_off) {
+      i0 = uniform(a.img_off[b]);
+      nimg = uniform(a.img_off[b + 1]) - i0;
+#pragma unroll
+      for (int q = 0; q < MAXIMG; ++q)
+        if ((uint32_t)q < nimg) img[q] = uniform(a.img[i0 + q]);
+    }
+  }
+
+  __device__ __forceinline__ void stage(u32x4 *t, int lane) const {
+    t[TL::at(lane >> 3, lane & 7)] = own;
+#pragma unroll
+    for (int n = 0; n < NF; ++n)
+      if (fdst[n] >= 0) t[fdst[n]] = fr[n];
+  }
+
+  __device__ __forceinline__ void compute(const StarArgs<bf16, R, COND> &a, const u32x4 *t,
+                                          int lane) const {
+    const int k = lane >> 3, j = lane & 7;
+    float c[6 * R + 1];
+#pragma unroll
+    for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
+    // west cells 8-R .. 7 and east cells 0 .. R-1
+    float xw[R], xe[R];
+    if constexpr (R <= 2) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        xw[q] = (q + 2 - R) ? bf16_hi(we) : bf16_lo(we);
+        xe[q] = q ? bf16_hi(ee) : bf16_lo(ee);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        xw[q] = (q & 1) ? bf16_hi(we[q >> 1]) : bf16_lo(we[q >> 1]);
+        xe[q] = (q & 1) ? bf16_hi(ee[q >> 1]) : bf16_lo(ee[q >> 1]);
+      }
+    }
+    float x[8], acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      x[i] = bf16_cell(own, i);
+      acc[i] = c[0] * tapv<float, COND>(x[i]);
+    }
+    auto X = [&](int i) -> float { return i < 0 ? xw[i + R] : (i >= 8 ? xe[i - 8] : x[i]); };
+#pragma unroll
+    for (int r = 1; r <= R; ++r) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        acc[i] = fmadd(c[6 * r - 5], tapv<float, COND>(X(i + r)), acc[i]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        acc[i] = fmadd(c[6 * r - 4], tapv<float, COND>(X(i - r)), acc[i]);
+      const u32x4 yp = t[TL::at(k, j + r)], ym = t[TL::at(k, j - r)];
+      const u32x4 zp = t[TL::at(k + r, j)], zm = t[TL::at(k - r, j)];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        acc[i] = fmadd(c[6 * r - 3], tapv<float, COND>(bf16_cell(yp, i)), acc[i]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        acc[i] = fmadd(c[6 * r - 2], tapv<float, COND>(bf16_cell(ym, i)), acc[i]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        acc[i] = fmadd(c[6 * r - 1], tapv<float, COND>(bf16_cell(zp, i)), acc[i]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        acc[i] = fmadd(c[6 * r], tapv<float, COND>(bf16_cell(zm, i)), acc[i]);
+    }
+    if constexpr (COND) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const bool sel = a.cond_mode == 0 ? (acc[i] > 0.0f) : (x[i] > a.theta);
+        acc[i] = sel ? acc[i] : -acc[i];
+      }
+    }
+    u32x4 o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = pack_bf16(acc[2 * q], acc[2 * q + 1]);
+    uint16_t *out = reinterpret_cast<uint16_t *>(a.out);
+    const size_t ost = a.out_step;
+    __builtin_nontemporal_store(o, reinterpret_cast<u32x4 *>(out + (size_t)b * ost + lane * 8));
+    // the fused refresh: the same row into each ghost image of b
+#pragma unroll
+    for (int q = 0; q < MAXIMG; ++q)
+      if ((uint32_t)q < nimg)
+        __builtin_nontemporal_store(
+            o, reinterpret_cast<u32x4 *>(out + (size_t)img[q] * ost + lane * 8));
+    for (uint32_t q = MAXIMG; q < nimg; ++q)
+      __builtin_nontemporal_store(
+          o, reinterpret_cast<u32x4 *>(out + (size_t)uniform(a.img[i0 + q]) * ost + lane * 8));
+  }
+};
+
+// WPB waves per workgroup, BPW bricks per wave: wave w of workgroup g takes
+// list entries (g' * WPB + w) * BPW + u, u < BPW (g' the XCD-remapped g).
+template <int R, bool COND, int WPB = 4, int BPW = BK_ROWS_BPW>
+__global__ __launch_bounds__(64 * WPB) void star_sweep_rows(const StarArgs<bf16, R, COND> a) {
+  using TL = RowTile<R>;
+  __shared__ u32x4 tile[WPB][BPW][TL::P * TL::P];
+
+  const int w = (int)uniform(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const uint32_t slot0 =
+      (xcd_remap(blockIdx.x, gridDim.x) * (uint32_t)WPB + (uint32_t)w) * (uint32_t)BPW;
+
+  RowBrick<R, COND> rb[BPW];
+#pragma unroll
+  for (int u = 0; u < BPW; ++u)
+    if (slot0 + u < a.nlist) rb[u].load(a, uniform(a.list[slot0 + u]), lane);
+#pragma unroll
+  for (int u = 0; u < BPW; ++u)
+    if (slot0 + u < a.nlist) rb[u].stage(tile[w][u], lane);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < BPW; ++u)
+    if (slot0 + u < a.nlist) rb[u].compute(a, tile[w][u], lane);
+}
 
 // The FP64 radius-4 column kernel stages its LDS tile in two halves of four
 // planes: a full 16 KB tile per one-wave workgroup held it at 2.5 waves per
@@ -1195,6 +1414,33 @@ hipError_t launch_cols_batched_dt(int stencil, const BatchTable *tab, uint32_t t
   }
 }
 
+template <int R, bool COND>
+hipError_t launch_rows(const StarArgs<bf16, R, COND> &a, uint32_t nlist, int wg_bricks,
+                       hipStream_t s) {
+  constexpr uint32_t B = BK_ROWS_BPW;
+  switch (wg_bricks) {
+    case 1:
+      hipLaunchKernelGGL((star_sweep_rows<R, COND, 1>), dim3((nlist + B - 1) / B), dim3(64), 0,
+                         s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((star_sweep_rows<R, COND, 2>), dim3((nlist + 2 * B - 1) / (2 * B)),
+                         dim3(128), 0, s, a);
+      break;
+    case 8:
+      hipLaunchKernelGGL((star_sweep_rows<R, COND, 8>), dim3((nlist + 8 * B - 1) / (8 * B)),
+                         dim3(512), 0, s, a);
+      break;
+    case 0:
+    case 4:
+      hipLaunchKernelGGL((star_sweep_rows<R, COND>), dim3((nlist + 4 * B - 1) / (4 * B)),
+                         dim3(256), 0, s, a);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <class T, int R, bool COND>
 hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   using A = typename acc_of<T>::type;
@@ -1213,6 +1459,13 @@ hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   a.nunits = a.kcol = 0;
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
+  if constexpr (std::is_same<T, bf16>::value) {
+    // the row form where fields and brick steps keep 16-byte rows aligned
+    const bool aligned = (reinterpret_cast<uintptr_t>(L.in) & 15u) == 0 &&
+                         (reinterpret_cast<uintptr_t>(L.out) & 15u) == 0 &&
+                         L.in_step % 8 == 0 && L.out_step % 8 == 0;
+    if (aligned) return launch_rows<R, COND>(a, L.nlist, L.wg_bricks, s);
+  }
   switch (L.wg_bricks) {
     case 1:
       hipLaunchKernelGGL((star_sweep_lds<T, R, COND, 1>), dim3(L.nlist), dim3(64), 0, s, a);

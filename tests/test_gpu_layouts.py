@@ -66,7 +66,7 @@ def test_888_star_bricks_per_workgroup(gpu, orc, name, wg):
     st = S.Stencil.make(name, seed=13)
     nb = (5, 3, 4)  # 60 interior bricks: not a multiple of 8
     og = O.Grid(nb)
-    for dt in (BK_F64, BK_F32):
+    for dt in (BK_F64, BK_F32, BK_BF16):
         print(f"[case] 8x8x8 {name} wg={wg} dt={dt}", flush=True)
         g, got = gpu_sweeps(gpu, nb, 0, 1, st, dt, 3, wg)
         oa, ob = og.storage(dt), og.storage(dt)

@@ -175,6 +175,40 @@ def test_interleaved_fields(gpu, stencil):
     np.testing.assert_array_equal(h[og.interior, :512], oa[og.interior])
 
 
+@pytest.mark.parametrize("stencil", [S.BK_STAR7, S.BK_STAR13, S.BK_STAR25, S.BK_COND7],
+                         ids=lambda s: NAMES[s])
+def test_bf16_row_form_and_unaligned_fields(gpu, stencil):
+    """BF16 per-brick sweeps move each i-row as one 16-byte load and store
+    when the fields and the brick step keep rows 16-byte aligned, and fall
+    back to the per-cell kernel otherwise (a step that is not a multiple of
+    8 elements, a field at an odd offset): the oracle's bits either way,
+    and the other field of the storage untouched."""
+    from bricklib_amd import BK_BF16
+    st = S.Stencil.make(stencil, seed=29)
+    nb = (3, 2, 3)
+    og = O.Grid(nb)
+    oa = og.storage(BK_BF16)
+    og.fill_uniform(oa, 37, dtype=BK_BF16)
+    og.refresh(oa, BK_BF16)
+    ob = og.storage(BK_BF16)
+    og.sweep(stencil, st.coeff, oa, ob, dtype=BK_BF16, theta=st.theta, cond_mode=st.cond_mode)
+    g = gpu.PeriodicGrid(nb)
+    for step, off_in, off_out in ((1024, 0, 512), (1028, 0, 514), (1040, 4, 520),
+                                  (1032, 520, 8)):
+        print(f"[case] bf16 rows {NAMES[stencil]} step={step} in@{off_in} out@{off_out}",
+              flush=True)
+        storage = gpu.BrickStorage.allocate(g.nbricks, step, BK_BF16)
+        storage.buf.zero()
+        bin_ = gpu.Brick(g.info, storage, off_in)
+        bout = gpu.Brick(g.info, storage, off_out)
+        g.fill_uniform(bin_, 37)
+        g.ghost_refresh(bin_)
+        gpu.sweep(st, bout, bin_, g)
+        h = storage.to_host()
+        np.testing.assert_array_equal(h[og.interior, off_out:off_out + 512], ob[og.interior])
+        np.testing.assert_array_equal(h[:, off_in:off_in + 512], oa)
+
+
 @pytest.mark.parametrize("nb", [(1, 1, 1), (1, 2, 5), (7, 3, 1)])
 def test_ragged_and_single_brick_grids(gpu, nb):
     for stencil in STENCILS:
