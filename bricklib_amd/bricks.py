@@ -344,11 +344,12 @@ class Plan:
 #   quad2, 25-point quad4 0.287 / 0.289 against 0.390 / 0.366.  One level
 #   (no k-march) wins for FP64 radius 1-2: four times the units of quad4,
 #   and the k-faces a march would save are a quarter of what the shared
-#   tile saves.  BF16 stars and cond: one brick per wave in the row form
-#   (16-byte rows; kernel alone / with a separate refresh: 7-point 0.131 /
-#   0.143 against quad8 0.140 / 0.152, 13-point 0.139 / 0.151 against
-#   quad2 0.159 / 0.173, 25-point 0.195 / 0.208 against quad2 0.206 /
-#   0.218, cond 0.146 / 0.158 against quad4 0.157 / 0.171).
+#   tile saves.  BF16 (the row-form kernels: 16-byte rows): 7/13-point and
+#   cond in 2x2 blocks of 4 levels (kernel alone / with a separate refresh:
+#   7-point 0.117 / 0.129 against one brick per wave 0.131 / 0.143 and one
+#   level 0.144 / 0.158; 13-point 0.131 / 0.144 against 0.139 / 0.151; cond
+#   0.131 / 0.145 against 0.146 / 0.158), the 25-point one brick per wave
+#   (0.180-0.184 / 0.192 against blocks of 8 levels 0.212 / 0.225).
 #   cube: quads of 4 levels (1-3% ahead of 8 levels at 512^3, whose
 #   smaller halo share is outweighed by the shorter tail of more units).
 DEFAULT_PLANS = {
@@ -356,6 +357,7 @@ DEFAULT_PLANS = {
     (2, BK_F64): ("quad", 2),
     (0, BK_F32): ("quad", 2), (1, BK_F32): ("quad", 2), (4, BK_F32): ("quad", 2),
     (2, BK_F32): ("quad", 4),
+    (0, BK_BF16): ("quad", 4), (1, BK_BF16): ("quad", 4), (4, BK_BF16): ("quad", 4),
     (3, BK_F64): ("quad", 4), (3, BK_F32): ("quad", 4), (3, BK_BF16): ("quad", 4),
 }
 # lane_shift of the quad star kernels per (stencil, dtype); absent = 0 (the

@@ -419,6 +419,13 @@ __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w 
 __device__ __forceinline__ float bf16_cell(const u32x4 &row, int i) {
   return (i & 1) ? bf16_hi(row[i >> 1]) : bf16_lo(row[i >> 1]);
 }
+// p[i] of a table the kernel never writes (adjacency, plans, image lists),
+// as a scalar load: in a loop that also stores, a plain load of it would be
+// a vector load waited for on its own.
+__device__ __forceinline__ uint32_t ld_table(const uint32_t *p, size_t i) {
+  return ((kconst_ptr<uint32_t>)p)[i];
+}
+
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
   bf16 a, b;
   narrow(a, lo);
@@ -440,13 +447,115 @@ struct RowTile {
 #define BK_ROWS_BPW 1
 #endif
 
+// The row form's arithmetic for lane (k, j): its own row of eight cells,
+// the i edges past the brick (as loaded: west cells 6, 7 / east cells 0, 1
+// for R <= 2, west 4 .. 7 / east 0 .. 3 for R = 4) and rows(dk, dj), the
+// row at (k + dk, j + dj); the taps in the column form's order.  Returns the
+// eight results rounded to BF16.
+template <int R, bool COND, class Edge, class Rows>
+__device__ __forceinline__ u32x4 row_star(const StarArgs<bf16, R, COND> &a, const u32x4 &own,
+                                          const Edge &we, const Edge &ee, Rows rows) {
+  float c[6 * R + 1];
+#pragma unroll
+  for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
+  // west cells 8-R .. 7 and east cells 0 .. R-1
+  float xw[R], xe[R];
+  if constexpr (R <= 2) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      xw[q] = (q + 2 - R) ? bf16_hi(we) : bf16_lo(we);
+      xe[q] = q ? bf16_hi(ee) : bf16_lo(ee);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      xw[q] = (q & 1) ? bf16_hi(we[q >> 1]) : bf16_lo(we[q >> 1]);
+      xe[q] = (q & 1) ? bf16_hi(ee[q >> 1]) : bf16_lo(ee[q >> 1]);
+    }
+  }
+  float x[8], acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    x[i] = bf16_cell(own, i);
+    acc[i] = c[0] * tapv<float, COND>(x[i]);
+  }
+  auto X = [&](int i) -> float { return i < 0 ? xw[i + R] : (i >= 8 ? xe[i - 8] : x[i]); };
+#pragma unroll
+  for (int r = 1; r <= R; ++r) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      acc[i] = fmadd(c[6 * r - 5], tapv<float, COND>(X(i + r)), acc[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      acc[i] = fmadd(c[6 * r - 4], tapv<float, COND>(X(i - r)), acc[i]);
+    const u32x4 yp = rows(0, r), ym = rows(0, -r), zp = rows(r, 0), zm = rows(-r, 0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      acc[i] = fmadd(c[6 * r - 3], tapv<float, COND>(bf16_cell(yp, i)), acc[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      acc[i] = fmadd(c[6 * r - 2], tapv<float, COND>(bf16_cell(ym, i)), acc[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      acc[i] = fmadd(c[6 * r - 1], tapv<float, COND>(bf16_cell(zp, i)), acc[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      acc[i] = fmadd(c[6 * r], tapv<float, COND>(bf16_cell(zm, i)), acc[i]);
+  }
+  if constexpr (COND) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const bool sel = a.cond_mode == 0 ? (acc[i] > 0.0f) : (x[i] > a.theta);
+      acc[i] = sel ? acc[i] : -acc[i];
+    }
+  }
+  u32x4 o;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = pack_bf16(acc[2 * q], acc[2 * q + 1]);
+  return o;
+}
+
+// Lane's row o of brick b, and of its nimg ghost images (the fused refresh):
+// ids img[0 .. 7) read ahead, any further ones from a.img[i0 + q].
+template <int R, bool COND>
+__device__ __forceinline__ void row_store(const StarArgs<bf16, R, COND> &a, const u32x4 &o,
+                                          uint32_t b, int lane, uint32_t nimg, uint32_t i0,
+                                          const uint32_t (&img)[7]) {
+  uint16_t *out = reinterpret_cast<uint16_t *>(a.out);
+  const size_t ost = a.out_step;
+  __builtin_nontemporal_store(o, reinterpret_cast<u32x4 *>(out + (size_t)b * ost + lane * 8));
+#pragma unroll
+  for (int q = 0; q < 7; ++q)
+    if ((uint32_t)q < nimg)
+      __builtin_nontemporal_store(o,
+                                  reinterpret_cast<u32x4 *>(out + (size_t)img[q] * ost + lane * 8));
+  for (uint32_t q = 7; q < nimg; ++q)
+    __builtin_nontemporal_store(
+        o, reinterpret_cast<u32x4 *>(out + (size_t)uniform(a.img[i0 + q]) * ost + lane * 8));
+}
+
+// the fused refresh's image ids of brick b: count, first index, the first
+// seven ids (scalar loads)
+template <int R, bool COND>
+__device__ __forceinline__ void row_images(const StarArgs<bf16, R, COND> &a, uint32_t b,
+                                           uint32_t &nimg, uint32_t &i0, uint32_t (&img)[7]) {
+  nimg = 0;
+  i0 = 0;
+  if (a.img_off) {
+    i0 = ld_table(a.img_off, b);
+    nimg = ld_table(a.img_off, (size_t)b + 1) - i0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+      if ((uint32_t)q < nimg) img[q] = ld_table(a.img, (size_t)i0 + q);
+  }
+}
+
 template <int R, bool COND>
 struct RowBrick {
   static constexpr int NF = (32 * R + 63) / 64;  // face rows per lane
-  static constexpr int MAXIMG = 7;  // image ids read ahead (a periodic grid's corner brick has 7)
   using TL = RowTile<R>;
   uint32_t b = 0, nimg = 0, i0 = 0;
-  uint32_t img[MAXIMG];
+  uint32_t img[7];  // image ids read ahead (a periodic grid's corner brick has 7)
   u32x4 own;
   u32x4 fr[NF];
   int fdst[NF];
@@ -460,9 +569,6 @@ struct RowBrick {
   // the fused refresh's image ids
   __device__ __forceinline__ void load(const StarArgs<bf16, R, COND> &a, uint32_t bb, int lane) {
     b = bb;
-    const int k = lane >> 3, j = lane & 7;
-    (void)k;
-    (void)j;
     const uint32_t *ab = a.adj + (size_t)b * 27;
     const uint16_t *in = reinterpret_cast<const uint16_t *>(a.in);
     const size_t st = a.in_step;
@@ -498,13 +604,7 @@ struct RowBrick {
     }
     we = *reinterpret_cast<const Edge *>(pim + (R <= 2 ? 6 : 4));
     ee = *reinterpret_cast<const Edge *>(pip);
-    if (a.img_off) {
-      i0 = uniform(a.img_off[b]);
-      nimg = uniform(a.img_off[b + 1]) - i0;
-#pragma unroll
-      for (int q = 0; q < MAXIMG; ++q)
-        if ((uint32_t)q < nimg) img[q] = uniform(a.img[i0 + q]);
-    }
+    row_images(a, b, nimg, i0, img);
   }
 
   __device__ __forceinline__ void stage(u32x4 *t, int lane) const {
@@ -517,76 +617,9 @@ struct RowBrick {
   __device__ __forceinline__ void compute(const StarArgs<bf16, R, COND> &a, const u32x4 *t,
                                           int lane) const {
     const int k = lane >> 3, j = lane & 7;
-    float c[6 * R + 1];
-#pragma unroll
-    for (int q = 0; q < 6 * R + 1; ++q) c[q] = a.c[q];
-    // west cells 8-R .. 7 and east cells 0 .. R-1
-    float xw[R], xe[R];
-    if constexpr (R <= 2) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        xw[q] = (q + 2 - R) ? bf16_hi(we) : bf16_lo(we);
-        xe[q] = q ? bf16_hi(ee) : bf16_lo(ee);
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        xw[q] = (q & 1) ? bf16_hi(we[q >> 1]) : bf16_lo(we[q >> 1]);
-        xe[q] = (q & 1) ? bf16_hi(ee[q >> 1]) : bf16_lo(ee[q >> 1]);
-      }
-    }
-    float x[8], acc[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      x[i] = bf16_cell(own, i);
-      acc[i] = c[0] * tapv<float, COND>(x[i]);
-    }
-    auto X = [&](int i) -> float { return i < 0 ? xw[i + R] : (i >= 8 ? xe[i - 8] : x[i]); };
-#pragma unroll
-    for (int r = 1; r <= R; ++r) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        acc[i] = fmadd(c[6 * r - 5], tapv<float, COND>(X(i + r)), acc[i]);
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        acc[i] = fmadd(c[6 * r - 4], tapv<float, COND>(X(i - r)), acc[i]);
-      const u32x4 yp = t[TL::at(k, j + r)], ym = t[TL::at(k, j - r)];
-      const u32x4 zp = t[TL::at(k + r, j)], zm = t[TL::at(k - r, j)];
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        acc[i] = fmadd(c[6 * r - 3], tapv<float, COND>(bf16_cell(yp, i)), acc[i]);
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        acc[i] = fmadd(c[6 * r - 2], tapv<float, COND>(bf16_cell(ym, i)), acc[i]);
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        acc[i] = fmadd(c[6 * r - 1], tapv<float, COND>(bf16_cell(zp, i)), acc[i]);
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        acc[i] = fmadd(c[6 * r], tapv<float, COND>(bf16_cell(zm, i)), acc[i]);
-    }
-    if constexpr (COND) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const bool sel = a.cond_mode == 0 ? (acc[i] > 0.0f) : (x[i] > a.theta);
-        acc[i] = sel ? acc[i] : -acc[i];
-      }
-    }
-    u32x4 o;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) o[q] = pack_bf16(acc[2 * q], acc[2 * q + 1]);
-    uint16_t *out = reinterpret_cast<uint16_t *>(a.out);
-    const size_t ost = a.out_step;
-    __builtin_nontemporal_store(o, reinterpret_cast<u32x4 *>(out + (size_t)b * ost + lane * 8));
-    // the fused refresh: the same row into each ghost image of b
-#pragma unroll
-    for (int q = 0; q < MAXIMG; ++q)
-      if ((uint32_t)q < nimg)
-        __builtin_nontemporal_store(
-            o, reinterpret_cast<u32x4 *>(out + (size_t)img[q] * ost + lane * 8));
-    for (uint32_t q = MAXIMG; q < nimg; ++q)
-      __builtin_nontemporal_store(
-          o, reinterpret_cast<u32x4 *>(out + (size_t)uniform(a.img[i0 + q]) * ost + lane * 8));
+    const u32x4 o = row_star<R, COND>(a, own, we, ee,
+                                      [&](int dk, int dj) { return t[TL::at(k + dk, j + dj)]; });
+    row_store<R, COND>(a, o, b, lane, nimg, i0, img);
   }
 };
 
@@ -613,6 +646,109 @@ __global__ __launch_bounds__(64 * WPB) void star_sweep_rows(const StarArgs<bf16,
 #pragma unroll
   for (int u = 0; u < BPW; ++u)
     if (slot0 + u < a.nlist) rb[u].compute(a, tile[w][u], lane);
+}
+
+// BF16 quad plans in the row form: the four waves of a workgroup take the
+// 2x2 block of bricks of a quad unit (member mm = wave: i-offset mm & 1,
+// j-offset mm >> 1), lane (k, j) of each member its brick's i-row (k, j).
+// The block's rows share one LDS tile of rows (kk, jj), kk = -R .. 7+R,
+// jj = -R .. 15+R, each of two 16-byte halves (left and right member), so
+// the j taps across the block's middle and the i edges between the two
+// members of a row come from the tile: each member loads its own 64 rows,
+// its two k faces and its outer j face (24R rows) and its outer i edge.
+// A unit's levels are swept one after the other, one barrier pair each.
+template <int R>
+struct QuadRowTile {
+  static constexpr int NK = 8 + 2 * R, NJ = 16 + 2 * R;
+  __device__ static constexpr int at(int kk, int jj, int half) {
+    return ((kk + R) * NJ + jj + R) * 2 + half;
+  }
+};
+
+template <int R, bool COND>
+__global__ __launch_bounds__(256) void star_sweep_quads_rows(const StarArgs<bf16, R, COND> a) {
+  using QT = QuadRowTile<R>;
+  using Edge = typename std::conditional<R <= 2, uint32_t, u32x2>::type;
+  constexpr int NF = (24 * R + 63) / 64;  // face rows per lane
+  __shared__ u32x4 tile[QT::NK * QT::NJ * 2];
+  const uint32_t u = xcd_remap(blockIdx.x, gridDim.x);
+  if (u >= a.nunits) return;  // uniform over the workgroup
+  const int mm = (int)uniform(threadIdx.x >> 6);
+  const int mi = mm & 1, mj = mm >> 1;
+  const int lane = threadIdx.x & 63;
+  const int k = lane >> 3, jy = 8 * mj + (lane & 7);  // this lane's tile row
+  const uint16_t *in = reinterpret_cast<const uint16_t *>(a.in);
+  const size_t st = a.in_step;
+  const int L = (int)ld_table(a.len, u);
+  // this member's level-0 brick and its outer i and j neighbours
+  uint32_t b = ld_table(a.anchor, u);
+  if (mi) b = ld_table(a.adj, (size_t)b * 27 + 14);
+  if (mj) b = ld_table(a.adj, (size_t)b * 27 + 16);
+  const int si = mi ? 14 : 12, sj = mj ? 16 : 10;
+  for (int l = 0; l < L; ++l) {
+    const size_t ab = (size_t)b * 27;
+    const bool store_me = ((ld_table(a.mask, (size_t)u * a.kcol + l) >> mm) & 1u) != 0;
+    // every global load before the first LDS store
+    const u32x4 own = *reinterpret_cast<const u32x4 *>(in + (size_t)b * st + lane * 8);
+    // face f / 8R: k-1, k+1, the outer j neighbour; row q = f % 8R of it
+    const uint16_t *pkm = in + (size_t)ld_table(a.adj, ab + 4) * st;
+    const uint16_t *pkp = in + (size_t)ld_table(a.adj, ab + 22) * st;
+    const uint16_t *pjo = in + (size_t)ld_table(a.adj, ab + sj) * st;
+    u32x4 fr[NF];
+    int fdst[NF];
+#pragma unroll
+    for (int n = 0; n < NF; ++n) {
+      const int f = n * 64 + lane;
+      fdst[n] = -1;
+      if (f < 24 * R) {
+        const int face = f / (8 * R), q = f % (8 * R);
+        int sk, sjj, kk, jj;
+        const uint16_t *src;
+        if (face < 2) {  // k faces: rows (kk, 8 mj + jj)
+          sjj = q & 7;
+          jj = 8 * mj + sjj;
+          sk = face == 0 ? 8 - R + (q >> 3) : q >> 3;
+          kk = face == 0 ? -R + (q >> 3) : 8 + (q >> 3);
+          src = face == 0 ? pkm : pkp;
+        } else {  // the outer j face: rows jj = -R .. -1 (mj = 0) or 16 .. 15 + R
+          sk = kk = q / R;
+          sjj = mj ? q % R : 8 - R + q % R;
+          jj = mj ? 16 + q % R : -R + q % R;
+          src = pjo;
+        }
+        fr[n] = *reinterpret_cast<const u32x4 *>(src + (sk * 8 + sjj) * 8);
+        fdst[n] = QT::at(kk, jj, mi);
+      }
+    }
+    // the outer i edge: the west neighbour's cells 6, 7 (4 .. 7) or the east
+    // neighbour's cells 0, 1 (0 .. 3) of the same row
+    const uint16_t *pio = in + (size_t)ld_table(a.adj, ab + si) * st + lane * 8;
+    const Edge eo = *reinterpret_cast<const Edge *>(pio + (mi ? 0 : (R <= 2 ? 6 : 4)));
+    uint32_t nimg = 0, i0 = 0, img[7];
+    if (store_me) row_images(a, b, nimg, i0, img);
+    tile[QT::at(k, jy, mi)] = own;
+#pragma unroll
+    for (int n = 0; n < NF; ++n)
+      if (fdst[n] >= 0) tile[fdst[n]] = fr[n];
+    __syncthreads();
+    if (store_me) {
+      // the inner i edge from the other member's half of this row
+      const u32x4 nb = tile[QT::at(k, jy, 1 - mi)];
+      Edge ei;
+      if constexpr (R <= 2) {
+        ei = mi ? nb[3] : nb[0];
+      } else {
+        ei = mi ? u32x2{nb[2], nb[3]} : u32x2{nb[0], nb[1]};
+      }
+      const u32x4 o = row_star<R, COND>(a, own, mi ? ei : eo, mi ? eo : ei,
+                                        [&](int dk, int dj) { return tile[QT::at(k + dk, jy + dj, mi)]; });
+      row_store<R, COND>(a, o, b, lane, nimg, i0, img);
+    }
+    if (l + 1 < L) {
+      __syncthreads();  // every wave's reads of this level's tile are done
+      b = ld_table(a.adj, ab + 22);
+    }
+  }
 }
 
 // The FP64 radius-4 column kernel stages its LDS tile in two halves of four
@@ -1178,6 +1314,13 @@ hipError_t launch_cols_t(const SweepLaunch &L, const PlanView &P, hipStream_t s)
   return hipGetLastError();
 }
 
+// BF16 fields whose 16-byte rows are 16-byte aligned: the row-form kernels
+bool rows_aligned(const SweepLaunch &L) {
+  return (reinterpret_cast<uintptr_t>(L.in) & 15u) == 0 &&
+         (reinterpret_cast<uintptr_t>(L.out) & 15u) == 0 && L.in_step % 8 == 0 &&
+         L.out_step % 8 == 0;
+}
+
 template <class T, int R, bool COND>
 hipError_t launch_quads_t(const SweepLaunch &L, const PlanView &P, hipStream_t s) {
   using A = typename acc_of<T>::type;
@@ -1202,6 +1345,13 @@ hipError_t launch_quads_t(const SweepLaunch &L, const PlanView &P, hipStream_t s
   a.theta = (A)L.theta;
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
   if (P.nunits == 0) return hipSuccess;
+  if constexpr (std::is_same<T, bf16>::value) {
+    // the row form where fields and brick steps keep 16-byte rows aligned
+    if (!L.lane_shift && rows_aligned(L)) {
+      hipLaunchKernelGGL((star_sweep_quads_rows<R, COND>), dim3(P.nunits), dim3(256), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   if (L.lane_shift)
     hipLaunchKernelGGL((star_sweep_quads<T, R, COND, true>), dim3(P.nunits), dim3(256), 0, s, a);
   else
@@ -1461,10 +1611,7 @@ hipError_t launch_t(const SweepLaunch &L, hipStream_t s) {
   for (int q = 0; q < 6 * R + 1; ++q) a.c[q] = (A)L.coeff[q];
   if constexpr (std::is_same<T, bf16>::value) {
     // the row form where fields and brick steps keep 16-byte rows aligned
-    const bool aligned = (reinterpret_cast<uintptr_t>(L.in) & 15u) == 0 &&
-                         (reinterpret_cast<uintptr_t>(L.out) & 15u) == 0 &&
-                         L.in_step % 8 == 0 && L.out_step % 8 == 0;
-    if (aligned) return launch_rows<R, COND>(a, L.nlist, L.wg_bricks, s);
+    if (rows_aligned(L)) return launch_rows<R, COND>(a, L.nlist, L.wg_bricks, s);
   }
   switch (L.wg_bricks) {
     case 1:

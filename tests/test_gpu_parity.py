@@ -178,11 +178,12 @@ def test_interleaved_fields(gpu, stencil):
 @pytest.mark.parametrize("stencil", [S.BK_STAR7, S.BK_STAR13, S.BK_STAR25, S.BK_COND7],
                          ids=lambda s: NAMES[s])
 def test_bf16_row_form_and_unaligned_fields(gpu, stencil):
-    """BF16 per-brick sweeps move each i-row as one 16-byte load and store
-    when the fields and the brick step keep rows 16-byte aligned, and fall
-    back to the per-cell kernel otherwise (a step that is not a multiple of
-    8 elements, a field at an odd offset): the oracle's bits either way,
-    and the other field of the storage untouched."""
+    """BF16 sweeps -- per brick, and in 2x2 blocks of 1 and 3 levels --
+    move each i-row as one 16-byte load and store when the fields and the
+    brick step keep rows 16-byte aligned, and fall back to the per-cell
+    kernels otherwise (a step that is not a multiple of 8 elements, a field
+    at an odd offset): the oracle's bits either way, and the other field of
+    the storage untouched."""
     from bricklib_amd import BK_BF16
     st = S.Stencil.make(stencil, seed=29)
     nb = (3, 2, 3)
@@ -195,18 +196,20 @@ def test_bf16_row_form_and_unaligned_fields(gpu, stencil):
     g = gpu.PeriodicGrid(nb)
     for step, off_in, off_out in ((1024, 0, 512), (1028, 0, 514), (1040, 4, 520),
                                   (1032, 520, 8)):
-        print(f"[case] bf16 rows {NAMES[stencil]} step={step} in@{off_in} out@{off_out}",
-              flush=True)
-        storage = gpu.BrickStorage.allocate(g.nbricks, step, BK_BF16)
-        storage.buf.zero()
-        bin_ = gpu.Brick(g.info, storage, off_in)
-        bout = gpu.Brick(g.info, storage, off_out)
-        g.fill_uniform(bin_, 37)
-        g.ghost_refresh(bin_)
-        gpu.sweep(st, bout, bin_, g)
-        h = storage.to_host()
-        np.testing.assert_array_equal(h[og.interior, off_out:off_out + 512], ob[og.interior])
-        np.testing.assert_array_equal(h[:, off_in:off_in + 512], oa)
+        for plan in (None, 1, 3):
+            print(f"[case] bf16 rows {NAMES[stencil]} step={step} in@{off_in} out@{off_out} "
+                  f"quad levels={plan}", flush=True)
+            storage = gpu.BrickStorage.allocate(g.nbricks, step, BK_BF16)
+            storage.buf.zero()
+            bin_ = gpu.Brick(g.info, storage, off_in)
+            bout = gpu.Brick(g.info, storage, off_out)
+            g.fill_uniform(bin_, 37)
+            g.ghost_refresh(bin_)
+            gpu.sweep(st, bout, bin_, g, plan=None if plan is None else g.plan("quad", plan))
+            h = storage.to_host()
+            np.testing.assert_array_equal(h[og.interior, off_out:off_out + 512],
+                                          ob[og.interior])
+            np.testing.assert_array_equal(h[:, off_in:off_in + 512], oa)
 
 
 @pytest.mark.parametrize("nb", [(1, 1, 1), (1, 2, 5), (7, 3, 1)])
@@ -435,12 +438,12 @@ def test_fused_refresh_quads_match_oracle(gpu, orc, name, kcol):
     and 5 sweeps is the oracle's (refresh every sweep), bit for bit, on odd
     grids (partial quads and levels), and the result's ghosts are its
     periodic images."""
-    from bricklib_amd import BK_F32
+    from bricklib_amd import BK_BF16, BK_F32
     st = S.Stencil.make(name, seed=33)
     for nb in ((3, 2, 5), (2, 2, 2)):
         g = gpu.PeriodicGrid(nb)
         og = O.Grid(nb)
-        for dt in (BK_F64, BK_F32):
+        for dt in (BK_F64, BK_F32, BK_BF16):
             for n in (1, 4, 5):
                 print(f"[case] fused quads {name} kcol={kcol} nb={nb} dt={dt} n={n}", flush=True)
                 a = gpu.Brick(g.info, g.field_storage(dt), 0)

@@ -167,6 +167,41 @@ def test_one_sweep_512_lane_shift_bit_exact(gpu, orc, kind, dtype):
     assert np.array_equal(got[pr[:, 0]], got[pr[:, 1]])
 
 
+@pytest.mark.parametrize("kind", [S.BK_STAR7, S.BK_STAR25, S.BK_COND7],
+                         ids=["7pt", "25pt", "cond"])
+@pytest.mark.parametrize("fused", [False, True], ids=["refresh", "fused"])
+@pytest.mark.parametrize("plan", [None, "quad"], ids=["brick", "quad4"])
+def test_two_sweeps_512_bf16_row_form_bit_exact(gpu, orc, kind, fused, plan):
+    """BF16 storage at full 512^3 through the row-form kernels (16-byte
+    rows; one brick per wave, and 2x2 blocks of 4 levels), two sweeps with
+    a separate or a fused refresh: the oracle's bits (FP32 FMAs in the fixed
+    order, round to nearest even on every store); with the fused refresh the
+    result's ghosts are its periodic images."""
+    from bricklib_amd import BK_BF16
+    print(f"[case] BF16 row form 2 sweeps 512^3 {NAMES[kind]} fused={fused} plan={plan}",
+          flush=True)
+    st = make_stencil(kind, 1, seed=8)
+    nb = (64, 64, 64)
+    g = gpu.PeriodicGrid(nb)
+    a = gpu.Brick(g.info, g.field_storage(BK_BF16), 0)
+    b = gpu.Brick(g.info, g.field_storage(BK_BF16), 0)
+    g.fill_uniform(a, 79)
+    b.storage.buf.zero()
+    res = gpu.sweeps(st, a, b, g, 2, fused=fused,
+                     plan=None if plan is None else g.plan("quad", 4))
+    got = res.storage.to_host()
+    og = O.Grid(nb)
+    oa = og.storage(BK_BF16)
+    og.fill_uniform(oa, 79, dtype=BK_BF16)
+    ob = og.storage(BK_BF16)
+    ores = og.sweeps(kind, st.coeff, oa, ob, 2, dtype=BK_BF16, theta=st.theta,
+                     cond_mode=st.cond_mode)
+    assert np.array_equal(got[og.interior], ores[og.interior])
+    if fused:
+        pr = g.pairs.reshape(-1, 2)
+        assert np.array_equal(got[pr[:, 0]], got[pr[:, 1]])
+
+
 # ---------------------------------------------------------------------------
 # closed forms on the device
 # ---------------------------------------------------------------------------
