@@ -685,6 +685,9 @@ __global__ __launch_bounds__(256) void star_sweep_quads_rows(const StarArgs<bf16
   if (mi) b = ld_table(a.adj, (size_t)b * 27 + 14);
   if (mj) b = ld_table(a.adj, (size_t)b * 27 + 16);
   const int si = mi ? 14 : 12, sj = mj ? 16 : 10;
+  // above the first level the k-1 face is the previous level's top R
+  // planes, still in the registers of lanes k >= 8 - R
+  u32x4 prev = {0u, 0u, 0u, 0u};
   for (int l = 0; l < L; ++l) {
     const size_t ab = (size_t)b * 27;
     const bool store_me = ((ld_table(a.mask, (size_t)u * a.kcol + l) >> mm) & 1u) != 0;
@@ -700,7 +703,7 @@ __global__ __launch_bounds__(256) void star_sweep_quads_rows(const StarArgs<bf16
     for (int n = 0; n < NF; ++n) {
       const int f = n * 64 + lane;
       fdst[n] = -1;
-      if (f < 24 * R) {
+      if (f < 24 * R && (l == 0 || f >= 8 * R)) {
         const int face = f / (8 * R), q = f % (8 * R);
         int sk, sjj, kk, jj;
         const uint16_t *src;
@@ -727,10 +730,12 @@ __global__ __launch_bounds__(256) void star_sweep_quads_rows(const StarArgs<bf16
     uint32_t nimg = 0, i0 = 0, img[7];
     if (store_me) row_images(a, b, nimg, i0, img);
     tile[QT::at(k, jy, mi)] = own;
+    if (l > 0 && k >= 8 - R) tile[QT::at(k - 8, jy, mi)] = prev;
 #pragma unroll
     for (int n = 0; n < NF; ++n)
       if (fdst[n] >= 0) tile[fdst[n]] = fr[n];
     __syncthreads();
+    prev = own;
     if (store_me) {
       // the inner i edge from the other member's half of this row
       const u32x4 nb = tile[QT::at(k, jy, 1 - mi)];
