@@ -350,15 +350,16 @@ class Plan:
 #   level 0.144 / 0.158; 13-point 0.131 / 0.144 against 0.139 / 0.151; cond
 #   0.131 / 0.145 against 0.146 / 0.158), the 25-point one brick per wave
 #   (0.180-0.184 / 0.192 against blocks of 8 levels 0.212 / 0.225).
-#   cube: quads of 4 levels (1-3% ahead of 8 levels at 512^3, whose
-#   smaller halo share is outweighed by the shorter tail of more units).
+#   cube: quads of 6 levels (kernel at 512^3: FP64 0.613 ms against 0.628
+#   with 4 levels, 0.646 with 2 and 0.635 with 8; FP32 0.514 against 0.520
+#   / 0.521 / 0.540).
 DEFAULT_PLANS = {
     (0, BK_F64): ("quad", 1), (1, BK_F64): ("quad", 1), (4, BK_F64): ("quad", 1),
     (2, BK_F64): ("quad", 2),
     (0, BK_F32): ("quad", 2), (1, BK_F32): ("quad", 2), (4, BK_F32): ("quad", 2),
     (2, BK_F32): ("quad", 4),
     (0, BK_BF16): ("quad", 4), (1, BK_BF16): ("quad", 4), (4, BK_BF16): ("quad", 4),
-    (3, BK_F64): ("quad", 4), (3, BK_F32): ("quad", 4), (3, BK_BF16): ("quad", 4),
+    (3, BK_F64): ("quad", 6), (3, BK_F32): ("quad", 6), (3, BK_BF16): ("quad", 6),
 }
 # lane_shift of the quad star kernels per (stencil, dtype); absent = 0 (the
 # i-taps from the LDS tile).
@@ -493,7 +494,8 @@ class PeriodicGrid:
         if int(stencil_kind) == _lib.BK_CUBE125 and self.interior.size > (1 << 20):
             # large grids: 8 levels (1024^3 FP64: 4.76 ms per sweep on one
             # host against 5.0-6.2 with 4; on most hosts every schedule lands
-            # at 5.8-6.1 ms, and 8 levels were never the slower)
+            # at 5.8-6.1 ms -- 4, 5, 6, 8 and 12 levels within 1.5 % of each
+            # other on one -- and 8 levels were never the slower)
             choice = ("quad", 8)
         return self.plan(*choice)
 
