@@ -652,16 +652,18 @@ __global__ __launch_bounds__(64 * WPB) void star_sweep_rows(const StarArgs<bf16,
 // 2x2 block of bricks of a quad unit (member mm = wave: i-offset mm & 1,
 // j-offset mm >> 1), lane (k, j) of each member its brick's i-row (k, j).
 // The block's rows share one LDS tile of rows (kk, jj), kk = -R .. 7+R,
-// jj = -R .. 15+R, each of two 16-byte halves (left and right member), so
-// the j taps across the block's middle and the i edges between the two
-// members of a row come from the tile: each member loads its own 64 rows,
-// its two k faces and its outer j face (24R rows) and its outer i edge.
-// A unit's levels are swept one after the other, one barrier pair each.
+// jj = -R .. 15+R, in two halves (the left and the right member's 16-byte
+// row parts, each half's rows contiguous so that a wave's row accesses are
+// consecutive 16-byte slots), so the j taps across the block's middle and
+// the i edges between the two members of a row come from the tile: each
+// member loads its own 64 rows, its two k faces and its outer j face (24R
+// rows) and its outer i edge.  A unit's levels are swept one after the
+// other, one barrier pair each.
 template <int R>
 struct QuadRowTile {
   static constexpr int NK = 8 + 2 * R, NJ = 16 + 2 * R;
   __device__ static constexpr int at(int kk, int jj, int half) {
-    return ((kk + R) * NJ + jj + R) * 2 + half;
+    return half * NK * NJ + (kk + R) * NJ + jj + R;
   }
 };
 
