@@ -419,7 +419,10 @@ class PeriodicGrid:
         # whole brick plane (T = 32 measured 4-6% faster for the 7-point star
         # at 512^3 than grid order; for the 2x2 block schedules, whose plans
         # follow this order, 32 x 16 is 0-3% ahead of 32 x 32 and 6-10% ahead
-        # of whole planes or 8 x 8).  Every brick is computed the same way in
+        # of whole planes or 8 x 8; 64 x 8, the same 512 bricks per tile
+        # plane, ran the radius-1/2 star kernels 0.5-1.3% faster alone but
+        # the fused 7-point step 1.9% slower, and 16 x 16, 16 x 8, 32 x 8 and
+        # 64 x 4 were 2-8% slower).  Every brick is computed the same way in
         # any order.
         self.tile = (tuple(int(x) for x in tile) if isinstance(tile, (tuple, list))
                      else ((int(tile), int(tile)) if tile else 0))
