@@ -212,15 +212,46 @@ def test_bf16_row_form_and_unaligned_fields(gpu, stencil):
             np.testing.assert_array_equal(h[:, off_in:off_in + 512], oa)
 
 
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32", "bf16"])
+def test_empty_brick_list_and_plan_are_no_ops(gpu, dtype):
+    """An empty brick list, and a plan built from one, sweep nothing: the
+    call succeeds, launches no kernel with an empty grid and leaves the
+    output untouched (per-brick, quad and column schedules, every stencil)."""
+    import ctypes as C
+    from bricklib_amd import _lib
+    from bricklib_amd.bricks import _desc
+    g, a, b, og, oa = setup(gpu, (2, 2, 3), dtype, 21)
+    empty = np.empty(0, np.uint32)
+    for stencil in STENCILS:
+        st = S.Stencil.make(stencil, seed=3)
+        for plan in (None, gpu.Plan(g.info, empty, "quad", 2),
+                     gpu.Plan(g.info, empty, "column", 4)):
+            print(f"[case] empty list {NAMES[stencil]} dtype={dtype} "
+                  f"plan={None if plan is None else plan.shape}", flush=True)
+            keep = []
+            d = _desc(st, b, a, g, keep, plan=plan)
+            if plan is None:
+                d.nlist = 0
+            _lib.check(_lib.load().bk_sweep(C.byref(d), None))
+            assert not host(b).any()
+
+
 @pytest.mark.parametrize("nb", [(1, 1, 1), (1, 2, 5), (7, 3, 1)])
 def test_ragged_and_single_brick_grids(gpu, nb):
-    for stencil in STENCILS:
-        st = S.Stencil.make(stencil, seed=6)
-        g, a, b, og, oa = setup(gpu, nb, 0, 2)
-        res = gpu.sweeps(st, a, b, g, 3)
-        ob = og.storage()
-        ores = og.sweeps(stencil, st.coeff, oa, ob, 3, theta=st.theta, cond_mode=st.cond_mode)
-        np.testing.assert_array_equal(host(res)[og.interior], ores[og.interior])
+    """Grids with one-brick axes (a brick that is its own neighbour, partial
+    2x2 blocks and levels) under the default schedules of every stencil, in
+    FP64 and in BF16 (whose defaults are the row-form kernels)."""
+    from bricklib_amd import BK_BF16
+    for dtype in (BK_F64, BK_BF16):
+        for stencil in STENCILS:
+            print(f"[case] ragged nb={nb} {NAMES[stencil]} dtype={dtype}", flush=True)
+            st = S.Stencil.make(stencil, seed=6)
+            g, a, b, og, oa = setup(gpu, nb, dtype, 2)
+            res = gpu.sweeps(st, a, b, g, 3)
+            ob = og.storage(dtype)
+            ores = og.sweeps(stencil, st.coeff, oa, ob, 3, dtype=dtype, theta=st.theta,
+                             cond_mode=st.cond_mode)
+            np.testing.assert_array_equal(host(res)[og.interior], ores[og.interior])
 
 
 def test_plain_case_128(gpu):
