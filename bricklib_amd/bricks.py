@@ -348,8 +348,9 @@ class Plan:
 #   cond in 2x2 blocks of 4 levels (kernel alone / with a separate refresh:
 #   7-point 0.117 / 0.129 against one brick per wave 0.131 / 0.143 and one
 #   level 0.144 / 0.158; 13-point 0.131 / 0.144 against 0.139 / 0.151; cond
-#   0.131 / 0.145 against 0.146 / 0.158), the 25-point one brick per wave
-#   (0.180-0.184 / 0.192 against blocks of 8 levels 0.212 / 0.225).
+#   0.131 / 0.145 against 0.146 / 0.158), the 25-point in blocks of 8
+#   levels (0.177-0.180 / 0.183-0.187 against one brick per wave 0.184-0.186
+#   / 0.186-0.187).
 #   cube: quads of 6 levels (kernel at 512^3: FP64 0.613 ms against 0.628
 #   with 4 levels, 0.646 with 2 and 0.635 with 8; FP32 0.514 against 0.520
 #   / 0.521 / 0.540).
@@ -359,6 +360,7 @@ DEFAULT_PLANS = {
     (0, BK_F32): ("quad", 2), (1, BK_F32): ("quad", 2), (4, BK_F32): ("quad", 2),
     (2, BK_F32): ("quad", 4),
     (0, BK_BF16): ("quad", 4), (1, BK_BF16): ("quad", 4), (4, BK_BF16): ("quad", 4),
+    (2, BK_BF16): ("quad", 8),
     (3, BK_F64): ("quad", 6), (3, BK_F32): ("quad", 6), (3, BK_BF16): ("quad", 6),
 }
 # lane_shift of the quad star kernels per (stencil, dtype); absent = 0 (the

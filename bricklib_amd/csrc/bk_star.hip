@@ -447,11 +447,17 @@ struct RowTile {
 #define BK_ROWS_BPW 1
 #endif
 
+// A tile row of eight BF16 cells.
+struct BRow {
+  u32x4 v;
+  __device__ __forceinline__ float cell(int i) const { return bf16_cell(v, i); }
+};
+
 // The row form's arithmetic for lane (k, j): its own row of eight cells,
 // the i edges past the brick (as loaded: west cells 6, 7 / east cells 0, 1
 // for R <= 2, west 4 .. 7 / east 0 .. 3 for R = 4) and rows(dk, dj), the
-// row at (k + dk, j + dj); the taps in the column form's order.  Returns the
-// eight results rounded to BF16.
+// row at (k + dk, j + dj) (a BRow); the taps in the column form's order.
+// Returns the eight results rounded to BF16.
 template <int R, bool COND, class Edge, class Rows>
 __device__ __forceinline__ u32x4 row_star(const StarArgs<bf16, R, COND> &a, const u32x4 &own,
                                           const Edge &we, const Edge &ee, Rows rows) {
@@ -488,19 +494,27 @@ __device__ __forceinline__ u32x4 row_star(const StarArgs<bf16, R, COND> &a, cons
 #pragma unroll
     for (int i = 0; i < 8; ++i)
       acc[i] = fmadd(c[6 * r - 4], tapv<float, COND>(X(i - r)), acc[i]);
-    const u32x4 yp = rows(0, r), ym = rows(0, -r), zp = rows(r, 0), zm = rows(-r, 0);
+    const auto yp = rows(0, r), ym = rows(0, -r), zp = rows(r, 0), zm = rows(-r, 0);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      acc[i] = fmadd(c[6 * r - 3], tapv<float, COND>(bf16_cell(yp, i)), acc[i]);
+      acc[i] = fmadd(c[6 * r - 3], tapv<float, COND>(yp.cell(i)), acc[i]);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      acc[i] = fmadd(c[6 * r - 2], tapv<float, COND>(bf16_cell(ym, i)), acc[i]);
+      acc[i] = fmadd(c[6 * r - 2], tapv<float, COND>(ym.cell(i)), acc[i]);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      acc[i] = fmadd(c[6 * r - 1], tapv<float, COND>(bf16_cell(zp, i)), acc[i]);
+      acc[i] = fmadd(c[6 * r - 1], tapv<float, COND>(zp.cell(i)), acc[i]);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      acc[i] = fmadd(c[6 * r], tapv<float, COND>(bf16_cell(zm, i)), acc[i]);
+      acc[i] = fmadd(c[6 * r], tapv<float, COND>(zm.cell(i)), acc[i]);
+    // radius 4 one radius at a time: left to itself the compiler reads
+    // every radius's rows first and then runs each cell's whole chain (the
+    // 2x2-block kernel in 111 VGPRs, 4 waves per SIMD: 25-point 211-220 us
+    // against 177-182 in 72); an empty asm that takes the accumulators and
+    // clobbers memory ends this radius's taps before the next rows are read
+    if constexpr (R == 4)
+      __asm__ volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]),
+                       "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7])::"memory");
   }
   if constexpr (COND) {
 #pragma unroll
@@ -617,8 +631,8 @@ struct RowBrick {
   __device__ __forceinline__ void compute(const StarArgs<bf16, R, COND> &a, const u32x4 *t,
                                           int lane) const {
     const int k = lane >> 3, j = lane & 7;
-    const u32x4 o = row_star<R, COND>(a, own, we, ee,
-                                      [&](int dk, int dj) { return t[TL::at(k + dk, j + dj)]; });
+    const u32x4 o = row_star<R, COND>(
+        a, own, we, ee, [&](int dk, int dj) { return BRow{t[TL::at(k + dk, j + dj)]}; });
     row_store<R, COND>(a, o, b, lane, nimg, i0, img);
   }
 };
@@ -747,8 +761,9 @@ __global__ __launch_bounds__(256) void star_sweep_quads_rows(const StarArgs<bf16
       } else {
         ei = mi ? u32x2{nb[2], nb[3]} : u32x2{nb[0], nb[1]};
       }
-      const u32x4 o = row_star<R, COND>(a, own, mi ? ei : eo, mi ? eo : ei,
-                                        [&](int dk, int dj) { return tile[QT::at(k + dk, jy + dj, mi)]; });
+      const u32x4 o = row_star<R, COND>(a, own, mi ? ei : eo, mi ? eo : ei, [&](int dk, int dj) {
+        return BRow{tile[QT::at(k + dk, jy + dj, mi)]};
+      });
       row_store<R, COND>(a, o, b, lane, nimg, i0, img);
     }
     if (l + 1 < L) {
