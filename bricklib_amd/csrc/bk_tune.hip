@@ -5,7 +5,7 @@
 // the 4x16 and the 8x8 fold), times every schedule the library has for that
 // layout -- one brick per wave with 1, 2, 4 or 8 bricks per workgroup,
 // k-columns of 4 / 8 / 16 bricks and 2x2 column quads of 1 / 2 / 4 / 8
-// levels for the stars, quads of 2 / 4 / 8 levels for the cube -- and returns them
+// levels for the stars, quads of 2 / 4 / 6 / 8 levels for the cube -- and returns them
 // fastest first.  The reference
 // has no tuner: its brick shape and fold are template arguments fixed at
 // compile time (brick.h:353-395, stencils/stencils.h:10-12) and its GPU
@@ -147,7 +147,7 @@ void candidates_of(int layout, int stencil, unsigned flags, std::vector<bk_tune_
   }
   if (stencil == BK_CUBE125) {
     if (!batch_only) add(0, 0, 4);
-    for (uint32_t k : {2u, 4u, 8u}) add(BK_PLAN_QUAD, k, 0);
+    for (uint32_t k : {2u, 4u, 6u, 8u}) add(BK_PLAN_QUAD, k, 0);
     return;
   }
   if (batch_only) {

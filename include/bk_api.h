@@ -422,7 +422,7 @@ typedef struct bk_tune_cand {
  * schedule of that layout -- per brick with 1, 2, 4 or 8 bricks per
  * workgroup (8: 8x8x8 stars only; 4: the cube on 8x8x8), k-columns of
  * 4/8/16 bricks and 2x2 column quads of 1/2/4/8 levels (8x8x8 stars; each
- * also with lane_shift = 1 unless BK_TUNE_BATCH_SCHEDULES), quads of 2/4/8
+ * also with lane_shift = 1 unless BK_TUNE_BATCH_SCHEDULES), quads of 2/4/6/8
  * levels (8x8x8 cube) -- untimed for warm_s seconds, then times `steps` ping-pong sweeps
  * per candidate (hipGraph replays) in `rounds` interleaved rounds, keeping
  * each candidate's best.  h_out receives up to max_out candidates fastest

@@ -124,7 +124,7 @@ def test_tuner_candidates_and_choice(gpu, orc, name):
     assert all(res[q].ms <= res[q + 1].ms for q in range(len(res) - 1))
     assert all(0.0 < c.ms < 1e3 for c in res)
     if name == "125pt":
-        assert {"8x8x8 quad2", "8x8x8 quad4", "8x8x8 quad8"} <= set(names)
+        assert {"8x8x8 quad2", "8x8x8 quad4", "8x8x8 quad6", "8x8x8 quad8"} <= set(names)
     else:
         assert {"8x8x8 brick wg1", "8x8x8 brick wg8", "8x8x8 column16"} <= set(names)
     # restricted to the batched kernels' schedules

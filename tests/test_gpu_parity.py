@@ -575,7 +575,7 @@ def test_tuner_picks_a_schedule_with_the_same_bits(gpu, name):
     before = gpu.sweeps(st, a, b, g, 3).storage.to_host()[g.interior]
     out = g.tune(st, BK_F64, steps=2, warm_s=0.05)
     if name == "125pt":
-        want = {"8x8x8 brick wg4", "8x8x8 quad2", "8x8x8 quad4", "8x8x8 quad8"}
+        want = {"8x8x8 brick wg4", "8x8x8 quad2", "8x8x8 quad4", "8x8x8 quad6", "8x8x8 quad8"}
     else:
         want = {"8x8x8 brick wg1", "8x8x8 brick wg2", "8x8x8 brick wg4", "8x8x8 brick wg8",
                 "8x8x8 column4", "8x8x8 column8", "8x8x8 column16", "8x8x8 quad1",
